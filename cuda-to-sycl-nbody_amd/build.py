@@ -1,0 +1,114 @@
+"""Builds libnbody_amd (HIP kernels + C ABI), libnbody_sim (C++ Simulator
+over the C ABI) and the nbody_amd driver, in-tree under lib/.
+
+    python cuda-to-sycl-nbody_amd/build.py
+
+Every HIP file is compiled for gfx950 only, with -ffp-contract=off (the
+integrator rounds as the reference source is written) and with packed fp32
+VALU instructions disabled.  After the build the gfx950 assembly is scanned
+and the build fails if any packed fp32 instruction was emitted.
+"""
+from __future__ import annotations
+
+import os
+import re
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+INCLUDE = ROOT / "include"
+CSRC = PKG / "csrc"
+HOST = PKG / "host"
+LIB = PKG / "lib"
+BUILD = ROOT / "_build"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+ARCH = "gfx950"
+
+HIP_SOURCES = ["nb_forward.hip", "nb_capi.hip"]
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-fPIC",
+    "-std=c++17",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-command-line-argument",
+    f"-I{INCLUDE}",
+    f"-I{CSRC}",
+]
+CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", f"-I{INCLUDE}"]
+
+# v_pk_{add,mul,fma}_f32 assembled from parts so no source names them whole.
+_PACKED_F32 = re.compile(r"\bv_" + "pk_" + r"(add|mul|fma)_f" + "32" + r"\b")
+
+
+def _run(cmd: list[str], cwd: Path | None = None) -> None:
+    print("+", " ".join(str(c) for c in cmd), flush=True)
+    res = subprocess.run([str(c) for c in cmd], cwd=cwd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout)
+        sys.stderr.write(res.stderr)
+        raise RuntimeError(f"build step failed ({res.returncode}): {cmd[0]}")
+    # hipcc's host pass warns that it does not know the gfx feature: drop it
+    noise = "is not a recognized feature for this target"
+    for line in (res.stdout + res.stderr).splitlines():
+        if line.strip() and noise not in line:
+            print(line)
+
+
+def _check_no_packed_fp32(asm_dir: Path) -> None:
+    bad = []
+    for s in sorted(asm_dir.glob(f"*-hip-amdgcn-amd-amdhsa-{ARCH}.s")):
+        text = s.read_text()
+        hits = _PACKED_F32.findall(text)
+        if hits:
+            bad.append(f"{s.name}: {len(hits)} packed fp32 ops")
+    if bad:
+        raise RuntimeError("packed fp32 instructions in device code: " + "; ".join(bad))
+
+
+def build(verbose: bool = True) -> Path:
+    """Compiles everything; returns the lib/ directory."""
+    if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    LIB.mkdir(parents=True, exist_ok=True)
+    obj_dir = BUILD / "obj"
+    obj_dir.mkdir(parents=True, exist_ok=True)
+
+    objs = []
+    for src in HIP_SOURCES:
+        obj = obj_dir / (Path(src).stem + ".o")
+        # --save-temps writes the gfx950 assembly next to the object (cwd)
+        _run([HIPCC, *HIP_FLAGS, "--save-temps", "-c", CSRC / src, "-o", obj], cwd=obj_dir)
+        objs.append(obj)
+    _check_no_packed_fp32(obj_dir)
+
+    ic_obj = obj_dir / "ic.o"
+    _run([CXX, *CXX_FLAGS, "-c", HOST / "ic.cpp", "-o", ic_obj])
+    objs.append(ic_obj)
+
+    lib = LIB / "libnbody_amd.so"
+    tmp = LIB / "libnbody_amd.so.tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
+    os.replace(tmp, lib)
+
+    sim = LIB / "libnbody_sim.so"
+    tmp = LIB / "libnbody_sim.so.tmp"
+    _run([CXX, *CXX_FLAGS, "-shared", HOST / "sim_param.cpp", HOST / "simulator.cpp",
+          f"-L{LIB}", "-lnbody_amd", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, sim)
+
+    drv = LIB / "nbody_amd"
+    _run([CXX, *CXX_FLAGS, HOST / "nbody_main.cpp", f"-L{LIB}", "-lnbody_sim", "-lnbody_amd",
+          "-Wl,-rpath,$ORIGIN", "-o", drv])
+    return LIB
+
+
+if __name__ == "__main__":
+    build()

@@ -1,0 +1,404 @@
+// C ABI of libnbody_amd: the simulator handle (DiskGalaxySimulator's device
+// side, src/simulator.cu:24-129) and the stream-ordered device entry points.
+// Nothing here throws across the boundary; HIP failures become status codes
+// with a per-thread message (the reference's gpuErrchk prints and exits
+// instead, src/simulator.cuh:22-31).
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <utility>
+
+#include "nb_internal.hpp"
+
+namespace nb {
+
+namespace {
+thread_local std::string g_last_error;
+} // namespace
+
+int set_error(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return NB_OK;
+  const int code =
+      (e == hipErrorOutOfMemory) ? NB_ERR_OUT_OF_MEMORY
+      : (e == hipErrorNoDevice || e == hipErrorInvalidDevice) ? NB_ERR_NO_DEVICE
+                                                               : NB_ERR_HIP;
+  return set_error(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int pair_mode(int calc_method, int acc, float eps) {
+  if (calc_method == NB_CALC_PREDICATED) return MODE_PREDICATED;
+  // The self term r * rsqrt(d^3) is exactly zero when d = eps gives a
+  // positive normal (or infinite) eps^3: then the test can be dropped.
+  const float e3 = (eps * eps) * eps;
+  const bool self_zero = (e3 >= FLT_MIN); // false for NaN, 0, <0, subnormal
+  if (acc == NB_ACC_KAHAN) return self_zero ? MODE_KAHAN : MODE_KAHAN_SKIP;
+  return self_zero ? MODE_PLAIN : MODE_PLAIN_SKIP;
+}
+
+namespace {
+struct DeviceScratch {
+  float *ptr = nullptr;
+  size_t bytes = 0;
+};
+constexpr int kMaxDevices = 64;
+DeviceScratch g_scratch[kMaxDevices];
+std::mutex g_scratch_mu;
+} // namespace
+
+float *device_scratch(size_t bytes, int *status) {
+  *status = NB_OK;
+  int dev = 0;
+  if ((*status = hip_check(hipGetDevice(&dev), "hipGetDevice"))) return nullptr;
+  if (dev < 0 || dev >= kMaxDevices) {
+    *status = set_error(NB_ERR_UNSUPPORTED, "device index too large");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  DeviceScratch &s = g_scratch[dev];
+  if (s.bytes < bytes) {
+    if (s.ptr) {
+      // hipFree waits for work that may still read the old buffer.
+      (void)hipFree(s.ptr);
+      s.ptr = nullptr;
+      s.bytes = 0;
+    }
+    void *p = nullptr;
+    if ((*status = hip_check(hipMalloc(&p, bytes), "scratch hipMalloc")))
+      return nullptr;
+    s.ptr = static_cast<float *>(p);
+    s.bytes = bytes;
+  }
+  return s.ptr;
+}
+
+} // namespace nb
+
+using namespace nb;
+
+struct nb_sim {
+  nb_params p{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  size_t n = 0, stride = 0;
+  float *mem = nullptr; // 9 SoA arrays: pos, pos_next, vel
+  float *pos[3] = {}, *nxt[3] = {}, *vel[3] = {};
+  float *scratch = nullptr;
+  int acc = NB_ACC_PLAIN;
+  int cfg = 0;
+  float last_ms = 0.0f, last_kernel_ms = 0.0f;
+};
+
+namespace {
+
+constexpr size_t kMaxParticles = (size_t)1 << 28; // int indices, 3 KiB-scale margins
+
+int check_params(const nb_params *p) {
+  if (!p) return set_error(NB_ERR_INVALID_ARG, "params is NULL");
+  if (p->calc_method != NB_CALC_BRANCH && p->calc_method != NB_CALC_PREDICATED)
+    return set_error(NB_ERR_INVALID_ARG,
+                     "Valid calculation methods are BRANCH or PREDICATED");
+  if (p->num_particles > kMaxParticles)
+    return set_error(NB_ERR_INVALID_ARG, "num_particles exceeds 2^28");
+  if (p->sim_iterations_per_frame < 0)
+    return set_error(NB_ERR_INVALID_ARG, "sim_iterations_per_frame < 0");
+  return NB_OK;
+}
+
+StepConsts consts_of(const nb_params *p) {
+  StepConsts c;
+  c.eps = p->dist_eps;
+  c.damping = p->damping;
+  c.dt = p->dt;
+  c.G = p->G;
+  return c;
+}
+
+void destroy_sim(nb_sim *s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->mem) (void)hipFree(s->mem);
+  if (s->scratch) (void)hipFree(s->scratch);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int ensure_sim_scratch(nb_sim *s) {
+  if (s->scratch) {
+    (void)hipFree(s->scratch);
+    s->scratch = nullptr;
+  }
+  const size_t bytes = forward_scratch_bytes(s->cfg, s->n);
+  if (bytes == 0) return NB_OK;
+  void *p = nullptr;
+  int st = hip_check(hipMalloc(&p, bytes), "scratch hipMalloc");
+  if (st) return st;
+  s->scratch = static_cast<float *>(p);
+  return NB_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+void nb_params_default(nb_params *p) {
+  if (!p) return;
+  p->G = 2.0f;
+  p->dt = 0.005f;
+  p->num_particles = 50 * 256;
+  p->num_frames = SIZE_MAX;
+  p->sim_iterations_per_frame = 4;
+  p->damping = 0.999998f;
+  p->dist_eps = 1.0e-7f;
+  p->gw_size = 64;
+  p->calc_method = NB_CALC_BRANCH;
+}
+
+const char *nb_last_error(void) { return g_last_error.c_str(); }
+
+int nb_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+int nb_device_name(char *buf, size_t len) {
+  if (!buf || len == 0) return set_error(NB_ERR_INVALID_ARG, "empty buffer");
+  hipDeviceProp_t prop;
+  const char *name = "Unknown Device";
+  if (hipGetDeviceProperties(&prop, 0) == hipSuccess) name = prop.name;
+  else (void)hipGetLastError();
+  std::strncpy(buf, name, len - 1);
+  buf[len - 1] = '\0';
+  return NB_OK;
+}
+
+int nb_sim_create(const nb_params *p, const float *px, const float *py,
+                  const float *pz, const float *vx, const float *vy,
+                  const float *vz, nb_sim **out) {
+  if (!out) return set_error(NB_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  int st = check_params(p);
+  if (st) return st;
+  const size_t n = p->num_particles;
+  if (n > 0 && (!px || !py || !pz || !vx || !vy || !vz))
+    return set_error(NB_ERR_INVALID_ARG, "particle array is NULL");
+  if (nb_device_count() == 0)
+    return set_error(NB_ERR_NO_DEVICE, "no HIP device visible");
+
+  nb_sim *s = new (std::nothrow) nb_sim;
+  if (!s) return set_error(NB_ERR_OUT_OF_MEMORY, "host allocation failed");
+  s->p = *p;
+  s->n = n;
+  s->stride = (n + 63) / 64 * 64; // every array starts 256 B aligned
+  s->cfg = forward_default_config(n, s->acc);
+  if ((st = hip_check(hipGetDevice(&s->device), "hipGetDevice")) ||
+      (st = hip_check(hipStreamCreateWithFlags(&s->stream,
+                                               hipStreamNonBlocking),
+                      "hipStreamCreate")) ||
+      (st = hip_check(hipEventCreate(&s->ev0), "hipEventCreate")) ||
+      (st = hip_check(hipEventCreate(&s->ev1), "hipEventCreate"))) {
+    destroy_sim(s);
+    return st;
+  }
+  if (n > 0) {
+    void *mem = nullptr;
+    st = hip_check(hipMalloc(&mem, sizeof(float) * 9 * s->stride),
+                   "particle hipMalloc");
+    if (st) {
+      destroy_sim(s);
+      return st;
+    }
+    s->mem = static_cast<float *>(mem);
+    for (int c = 0; c < 3; ++c) {
+      s->pos[c] = s->mem + (0 + c) * s->stride;
+      s->nxt[c] = s->mem + (3 + c) * s->stride;
+      s->vel[c] = s->mem + (6 + c) * s->stride;
+    }
+    const float *src[6] = {px, py, pz, vx, vy, vz};
+    float *dst[6] = {s->pos[0], s->pos[1], s->pos[2],
+                     s->vel[0], s->vel[1], s->vel[2]};
+    for (int k = 0; k < 6 && !st; ++k)
+      st = hip_check(hipMemcpyAsync(dst[k], src[k], sizeof(float) * n,
+                                    hipMemcpyHostToDevice, s->stream),
+                     "host-to-device copy");
+    if (!st) st = hip_check(hipStreamSynchronize(s->stream), "sync");
+    if (!st) st = ensure_sim_scratch(s);
+    if (st) {
+      destroy_sim(s);
+      return st;
+    }
+  }
+  *out = s;
+  return NB_OK;
+}
+
+int nb_sim_advance(nb_sim *s, int iters) {
+  if (!s) return set_error(NB_ERR_INVALID_ARG, "sim is NULL");
+  if (iters < 0) return set_error(NB_ERR_INVALID_ARG, "iters < 0");
+  if (s->n == 0) return NB_OK;
+  const StepConsts c = consts_of(&s->p);
+  const int mode = pair_mode(s->p.calc_method, s->acc, s->p.dist_eps);
+  for (int k = 0; k < iters; ++k) {
+    int st = launch_forward(c, mode, s->cfg, s->pos[0], s->pos[1], s->pos[2],
+                            s->vel[0], s->vel[1], s->vel[2], s->nxt[0],
+                            s->nxt[1], s->nxt[2], (int)s->n, s->scratch,
+                            s->stream);
+    if (st) return st;
+    // ping-pong of the position buffers (src/simulator.cu:65)
+    for (int d = 0; d < 3; ++d) std::swap(s->pos[d], s->nxt[d]);
+  }
+  return NB_OK;
+}
+
+int nb_sim_sync(nb_sim *s) {
+  if (!s) return set_error(NB_ERR_INVALID_ARG, "sim is NULL");
+  return hip_check(hipStreamSynchronize(s->stream), "stream synchronize");
+}
+
+int nb_sim_step(nb_sim *s) {
+  if (!s) return set_error(NB_ERR_INVALID_ARG, "sim is NULL");
+  const auto t0 = std::chrono::steady_clock::now();
+  int st = hip_check(hipEventRecord(s->ev0, s->stream), "hipEventRecord");
+  if (!st) st = nb_sim_advance(s, s->p.sim_iterations_per_frame);
+  if (!st) st = hip_check(hipEventRecord(s->ev1, s->stream), "hipEventRecord");
+  if (!st) st = hip_check(hipStreamSynchronize(s->stream), "stream synchronize");
+  const auto t1 = std::chrono::steady_clock::now();
+  if (st) return st;
+  s->last_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, s->ev0, s->ev1) == hipSuccess)
+    s->last_kernel_ms = ms;
+  else
+    (void)hipGetLastError();
+  return NB_OK;
+}
+
+static int read3(nb_sim *s, float *const src[3], float *x, float *y,
+                 float *z) {
+  if (!s) return set_error(NB_ERR_INVALID_ARG, "sim is NULL");
+  if (s->n == 0) return NB_OK;
+  if (!x || !y || !z) return set_error(NB_ERR_INVALID_ARG, "output is NULL");
+  float *dst[3] = {x, y, z};
+  for (int c = 0; c < 3; ++c) {
+    int st = hip_check(hipMemcpyAsync(dst[c], src[c], sizeof(float) * s->n,
+                                      hipMemcpyDeviceToHost, s->stream),
+                       "device-to-host copy");
+    if (st) return st;
+  }
+  return hip_check(hipStreamSynchronize(s->stream), "stream synchronize");
+}
+
+int nb_sim_read_pos(nb_sim *s, float *x, float *y, float *z) {
+  return s ? read3(s, s->pos, x, y, z)
+           : set_error(NB_ERR_INVALID_ARG, "sim is NULL");
+}
+
+int nb_sim_read_vel(nb_sim *s, float *x, float *y, float *z) {
+  return s ? read3(s, s->vel, x, y, z)
+           : set_error(NB_ERR_INVALID_ARG, "sim is NULL");
+}
+
+float nb_sim_last_step_ms(const nb_sim *s) { return s ? s->last_ms : 0.0f; }
+float nb_sim_last_kernel_ms(const nb_sim *s) {
+  return s ? s->last_kernel_ms : 0.0f;
+}
+size_t nb_sim_num_particles(const nb_sim *s) { return s ? s->n : 0; }
+int nb_sim_calc_method(const nb_sim *s) { return s ? s->p.calc_method : 0; }
+int nb_sim_gw_size(const nb_sim *s) { return s ? s->p.gw_size : 0; }
+
+int nb_sim_set_accumulation(nb_sim *s, int mode) {
+  if (!s) return set_error(NB_ERR_INVALID_ARG, "sim is NULL");
+  if (mode != NB_ACC_PLAIN && mode != NB_ACC_KAHAN)
+    return set_error(NB_ERR_INVALID_ARG, "unknown accumulation mode");
+  s->acc = mode;
+  return NB_OK;
+}
+
+int nb_sim_set_frame_iterations(nb_sim *s, int iters) {
+  if (!s) return set_error(NB_ERR_INVALID_ARG, "sim is NULL");
+  if (iters < 0) return set_error(NB_ERR_INVALID_ARG, "iters < 0");
+  s->p.sim_iterations_per_frame = iters;
+  return NB_OK;
+}
+
+int nb_sim_set_config(nb_sim *s, int cfg) {
+  if (!s) return set_error(NB_ERR_INVALID_ARG, "sim is NULL");
+  if (cfg < -1 || cfg >= forward_config_count())
+    return set_error(NB_ERR_INVALID_ARG, "forward config out of range");
+  int st = hip_check(hipStreamSynchronize(s->stream), "stream synchronize");
+  if (st) return st;
+  s->cfg = cfg < 0 ? forward_default_config(s->n, s->acc) : cfg;
+  return s->n ? ensure_sim_scratch(s) : NB_OK;
+}
+
+int nb_sim_config(const nb_sim *s) { return s ? s->cfg : -1; }
+
+int nb_sim_destroy(nb_sim *s) {
+  destroy_sim(s);
+  return NB_OK;
+}
+
+int nb_forward_step_dev(const nb_params *p, const float *px, const float *py,
+                        const float *pz, float *vx, float *vy, float *vz,
+                        float *nx, float *ny, float *nz, size_t n, int cfg,
+                        int acc, void *stream) {
+  int st = check_params(p);
+  if (st) return st;
+  if (n > kMaxParticles)
+    return set_error(NB_ERR_INVALID_ARG, "n exceeds 2^28");
+  if (acc != NB_ACC_PLAIN && acc != NB_ACC_KAHAN)
+    return set_error(NB_ERR_INVALID_ARG, "unknown accumulation mode");
+  if (cfg < -1 || cfg >= forward_config_count())
+    return set_error(NB_ERR_INVALID_ARG, "forward config out of range");
+  if (n == 0) return NB_OK;
+  if (!px || !py || !pz || !vx || !vy || !vz || !nx || !ny || !nz)
+    return set_error(NB_ERR_INVALID_ARG, "device pointer is NULL");
+  if (px == nx || py == ny || pz == nz)
+    return set_error(NB_ERR_INVALID_ARG,
+                     "positions and next positions must not alias");
+  if (cfg < 0) cfg = forward_default_config(n, acc);
+  float *scratch = nullptr;
+  const size_t bytes = forward_scratch_bytes(cfg, n);
+  if (bytes) {
+    scratch = device_scratch(bytes, &st);
+    if (st) return st;
+  }
+  return launch_forward(consts_of(p),
+                        pair_mode(p->calc_method, acc, p->dist_eps), cfg, px,
+                        py, pz, vx, vy, vz, nx, ny, nz, (int)n, scratch,
+                        static_cast<hipStream_t>(stream));
+}
+
+int nb_forward_config_count(void) { return forward_config_count(); }
+
+int nb_forward_config_info(int cfg, int *block, int *unroll, int *tile,
+                           int *split) {
+  if (cfg < 0 || cfg >= forward_config_count())
+    return set_error(NB_ERR_INVALID_ARG, "forward config out of range");
+  const FwdConfig &c = forward_config(cfg);
+  if (block) *block = c.block;
+  if (unroll) *unroll = c.unroll;
+  if (tile) *tile = c.tile;
+  if (split) *split = c.split;
+  return NB_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,71 @@
+// Internal declarations shared by the HIP translation units of libnbody_amd.
+// Not part of the public ABI (include/nbody_amd.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "nbody_amd.h"
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+namespace nb {
+
+// Records `msg` as the calling thread's last error and returns `code`.
+int set_error(int code, const std::string &msg);
+// Checks a HIP status; on failure records "<what>: <hip message>".
+int hip_check(hipError_t e, const char *what);
+
+// Physical step constants, in the order the kernels consume them.
+struct StepConsts {
+  float eps;     // SimParam::distEps
+  float damping; // SimParam::damping
+  float dt;      // SimParam::dt
+  float G;       // SimParam::G
+};
+
+// Pair-loop variants.  The self term of the reference's BRANCH kernel is
+// skipped with `continue` (src/simulator.cu:206); when dist_eps > 0 and
+// dist_eps^3 is a normal float the self term is exactly +0 (r = 0, finite
+// 1/d^1.5), so the fast variants omit the test.  The *_SKIP variants mask
+// j == i for the remaining parameter values.
+enum PairMode : int {
+  MODE_PLAIN = 0,
+  MODE_PLAIN_SKIP = 1,
+  MODE_KAHAN = 2,
+  MODE_KAHAN_SKIP = 3,
+  MODE_PREDICATED = 4,
+};
+
+// One compiled launch shape of the forward kernel.
+struct FwdConfig {
+  int block;  // threads per workgroup
+  int unroll; // target particles per thread
+  int tile;   // source particles staged in LDS per pass
+  int split;  // source-range chunks (grid.y); >1 writes partial forces
+};
+
+int forward_config_count();
+const FwdConfig &forward_config(int cfg);
+// Library's launch shape for n particles.
+int forward_default_config(size_t n, int acc);
+
+// Stream-ordered launch of one step on device SoA buffers.  `scratch` must
+// hold 3 * split * n floats when the shape has split > 1 (else unused).
+int launch_forward(const StepConsts &c, int mode, int cfg, const float *px,
+                   const float *py, const float *pz, float *vx, float *vy,
+                   float *vz, float *nx, float *ny, float *nz, int n,
+                   float *scratch, hipStream_t stream);
+
+// Bytes of scratch launch_forward needs for (cfg, n).
+size_t forward_scratch_bytes(int cfg, size_t n);
+
+// Per-device scratch owned by the library (grown on demand, never shrunk).
+// Calls on different streams of one device share it.
+float *device_scratch(size_t bytes, int *status);
+
+// Pair mode for a parameter set.
+int pair_mode(int calc_method, int acc, float eps);
+
+} // namespace nb

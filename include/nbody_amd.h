@@ -1,0 +1,175 @@
+/*
+ * nbody_amd.h - C ABI of the MI355X (gfx950) all-pairs gravity step.
+ *
+ * This is the drop-in boundary for the reference's simulation layer:
+ *   - class simulation::DiskGalaxySimulator  (src/simulator.cuh:129-160)
+ *   - kernel particle_interaction<CalculationMethod>  (src/simulator.cu:186-229)
+ *   - SimParam  (src/sim_param.hpp:16-39)
+ *
+ * Everything is plain C: pointers, sizes and status codes.  No HIP or torch
+ * type appears in a signature; streams are passed as `void *` (a hipStream_t,
+ * or NULL for the null stream).  Functions never throw and never exit: a
+ * nonzero status is returned and nb_last_error() holds a message for the
+ * calling thread.
+ *
+ * Arrays are structure-of-arrays (SoA): one float[n] per coordinate, the
+ * layout of ParticleData / ParticleData_d (src/simulator.cuh:75-98).
+ *
+ * Two levels:
+ *   nb_sim_*     an owning simulator handle: host arrays in, host arrays out,
+ *                device-resident state between steps (the DiskGalaxySimulator
+ *                contract).
+ *   nb_*_dev     stream-ordered launches on caller-owned device buffers, for
+ *                callers that manage device memory themselves (PyTorch, graphs).
+ */
+#ifndef NBODY_AMD_H
+#define NBODY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define NB_OK 0
+#define NB_ERR_INVALID_ARG 1   /* bad size, null pointer, unknown option       */
+#define NB_ERR_NO_DEVICE 2     /* no HIP device visible                         */
+#define NB_ERR_HIP 3           /* a HIP runtime call failed                     */
+#define NB_ERR_OUT_OF_MEMORY 4 /* device or host allocation failed              */
+#define NB_ERR_UNSUPPORTED 5   /* option combination not implemented            */
+
+/* CalculationMethod (src/sim_param.hpp:8-11).  PREDICATED reproduces the
+ * mounted reference exactly: force += r*inv*(j == id) (src/simulator.cu:209),
+ * which keeps only the self term. */
+#define NB_CALC_BRANCH 0
+#define NB_CALC_PREDICATED 1
+
+/* Force-sum accumulation mode (extension; BRANCH only). */
+#define NB_ACC_PLAIN 0 /* one fp32 FMA per term (default)                      */
+#define NB_ACC_KAHAN 1 /* Kahan-compensated fp32 sum of the per-particle force */
+
+/* SimParam, field for field (src/sim_param.hpp:30-39). */
+typedef struct nb_params {
+  float G;                    /* gravitational parameter                    */
+  float dt;                   /* time step                                  */
+  size_t num_particles;       /* N                                          */
+  size_t num_frames;          /* frames for a driver loop                   */
+  int sim_iterations_per_frame; /* kernel launches per stepSim()            */
+  float damping;              /* velocity drag factor (1 = none)            */
+  float dist_eps;             /* softening added to |r|^2                   */
+  int gw_size;                /* reference work-group size (a hint here)    */
+  int calc_method;            /* NB_CALC_BRANCH or NB_CALC_PREDICATED       */
+} nb_params;
+
+/* SimParam::SimParam() defaults (src/sim_param.cpp:12-22). */
+void nb_params_default(nb_params *p);
+
+/* Message for the last failed call on this thread ("" if none). */
+const char *nb_last_error(void);
+
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int nb_device_count(void);
+
+/* Name of device 0 (DiskGalaxySimulator::getDeviceName, src/simulator.cu:36-45).
+ * Writes "Unknown Device" when the query fails, as the reference does. */
+int nb_device_name(char *buf, size_t len);
+
+/* ---- initial conditions (host) ------------------------------------------ */
+/* Disk in the draw order of DiskGalaxySimulator::randomParticlePos and
+ * initialParticleVel (src/simulator.cu:131-158): per particle t then s, then
+ * all z; std::mt19937(seed) with uniform_real_distribution<double>.
+ * seed 5489 is the reference's default-constructed engine.
+ * flags: NB_IC_TRIG_DOUBLE evaluates cos/sin in double (the default is the
+ * float overload); NB_IC_LENGTH_F32 uses the fp32 length() of the SYCL twin
+ * (src_sycl/simulator.dp.cpp:299-301) instead of the double pow form
+ * (src/simulator.cu:170-172). */
+#define NB_IC_TRIG_DOUBLE 1
+#define NB_IC_LENGTH_F32 2
+int nb_ic_disk(size_t n, uint32_t seed, int flags, float *px, float *py,
+               float *pz, float *vx, float *vy, float *vz);
+
+/* Disk in the draw order of gen.cpp randomParticlePos/randomParticleVel
+ * (src/gen.cpp:15-34): t, s, z drawn per particle; glm-style normalize. */
+int nb_ic_disk_gen(size_t n, uint32_t seed, float *px, float *py, float *pz,
+                   float *vx, float *vy, float *vz);
+
+/* Plummer sphere (no reference counterpart): scale radius a, unit masses,
+ * isotropic velocities in virial equilibrium for gravitational parameter
+ * g_mass = G * total mass, sampled as in Aarseth, Henon & Wielen (1974),
+ * shifted to the centre-of-mass frame. */
+int nb_ic_plummer(size_t n, uint32_t seed, float a, float g_mass, float *px,
+                  float *py, float *pz, float *vx, float *vy, float *vz);
+
+/* ---- owning simulator handle (DiskGalaxySimulator) ---------------------- */
+typedef struct nb_sim nb_sim;
+
+/* Copies the caller's SoA arrays (p->num_particles each) to the device.
+ * Replaces the DiskGalaxySimulator(SimParam) constructor's sendToDevice
+ * (src/simulator.cu:24-34, 79-103) for caller-supplied initial conditions. */
+int nb_sim_create(const nb_params *p, const float *px, const float *py,
+                  const float *pz, const float *vx, const float *vy,
+                  const float *vz, nb_sim **out);
+
+/* stepSim() without the readback (src/simulator.cu:47-72): launches
+ * p->sim_iterations_per_frame steps, ping-pongs positions, synchronises and
+ * records the wall-clock frame time. */
+int nb_sim_step(nb_sim *s);
+
+/* Launches `iters` steps without synchronising (stream-ordered). */
+int nb_sim_advance(nb_sim *s, int iters);
+
+/* Blocks until every launched step has finished. */
+int nb_sim_sync(nb_sim *s);
+
+/* recvFromDevice (src/simulator.cu:106-129): current positions / velocities. */
+int nb_sim_read_pos(nb_sim *s, float *x, float *y, float *z);
+int nb_sim_read_vel(nb_sim *s, float *x, float *y, float *z);
+
+/* getLastStepTime (src/simulator.cuh:134): ms of the last nb_sim_step, from
+ * before the first launch until the device synchronise. */
+float nb_sim_last_step_ms(const nb_sim *s);
+
+/* Device time of the last nb_sim_step's launches, from HIP events (ms). */
+float nb_sim_last_kernel_ms(const nb_sim *s);
+
+size_t nb_sim_num_particles(const nb_sim *s); /* getNumParticles */
+int nb_sim_calc_method(const nb_sim *s);      /* getCM             */
+int nb_sim_gw_size(const nb_sim *s);          /* getGwSize         */
+
+/* Selects the accumulation mode (NB_ACC_*) used by later steps. */
+int nb_sim_set_accumulation(nb_sim *s, int mode);
+
+/* Changes simIterationsPerFrame for later nb_sim_step calls. */
+int nb_sim_set_frame_iterations(nb_sim *s, int iters);
+
+/* Selects a forward launch shape (see nb_forward_config_*) for later steps;
+ * -1 restores the library's choice. */
+int nb_sim_set_config(nb_sim *s, int cfg);
+int nb_sim_config(const nb_sim *s);
+
+int nb_sim_destroy(nb_sim *s);
+
+/* ---- stream-ordered device entry points --------------------------------- */
+/* One launch of particle_interaction (src/simulator.cu:186-229):
+ *   reads pos (px,py,pz), updates vel (vx,vy,vz) in place, writes the next
+ *   positions (nx,ny,nz).  pos and next must not overlap.  n = particle count.
+ * cfg selects a tuned launch shape (-1 = the library's choice for n);
+ * acc is NB_ACC_*. */
+int nb_forward_step_dev(const nb_params *p, const float *px, const float *py,
+                        const float *pz, float *vx, float *vy, float *vz,
+                        float *nx, float *ny, float *nz, size_t n, int cfg,
+                        int acc, void *stream);
+
+/* Number of compiled forward launch shapes, and a description of one:
+ * block threads, targets per thread, LDS tile length, j-split count. */
+int nb_forward_config_count(void);
+int nb_forward_config_info(int cfg, int *block, int *unroll, int *tile,
+                           int *split);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBODY_AMD_H */
