@@ -182,10 +182,19 @@ int nb_device_count(void) {
 int nb_device_name(char *buf, size_t len) {
   if (!buf || len == 0) return set_error(NB_ERR_INVALID_ARG, "empty buffer");
   hipDeviceProp_t prop;
-  const char *name = "Unknown Device";
-  if (hipGetDeviceProperties(&prop, 0) == hipSuccess) name = prop.name;
-  else (void)hipGetLastError();
-  std::strncpy(buf, name, len - 1);
+  std::string name = "Unknown Device";
+  if (hipGetDeviceProperties(&prop, 0) == hipSuccess) {
+    name = prop.name;
+    // Without the marketing-name table the runtime reports a generic name;
+    // the ISA name identifies the part.
+    const std::string arch = prop.gcnArchName;
+    const std::string isa = arch.substr(0, arch.find(':'));
+    if (!isa.empty() && name.find(isa) == std::string::npos)
+      name += " [" + isa + "]";
+  } else {
+    (void)hipGetLastError();
+  }
+  std::strncpy(buf, name.c_str(), len - 1);
   buf[len - 1] = '\0';
   return NB_OK;
 }

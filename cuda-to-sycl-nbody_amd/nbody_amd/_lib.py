@@ -71,6 +71,14 @@ def load() -> ctypes.CDLL:
     global _lib
     if _lib is not None:
         return _lib
+    # One HIP runtime per process: PyTorch loads its bundled libamdhip64 by
+    # path, and a second runtime loaded first (this library's /opt/rocm one)
+    # would not be shared with it.  Importing torch first makes this
+    # library's libamdhip64.so.7 dependency resolve to torch's copy.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not LIB_PATH.exists():
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with `python cuda-to-sycl-nbody_amd/build.py` "
