@@ -29,7 +29,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["nb_forward.hip", "nb_capi.hip"]
+HIP_SOURCES = ["nb_forward.hip", "nb_adjoint.hip", "nb_traj.hip", "nb_batch.hip", "nb_capi.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
@@ -44,8 +44,6 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", f"-I{INCLUDE}"]
 
-# v_pk_{add,mul,fma}_f32 assembled from parts so no source names them whole.
-_PACKED_F32 = re.compile(r"\bv_" + "pk_" + r"(add|mul|fma)_f" + "32" + r"\b")
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:
@@ -62,17 +60,6 @@ def _run(cmd: list[str], cwd: Path | None = None) -> None:
             print(line)
 
 
-def _check_no_packed_fp32(asm_dir: Path) -> None:
-    bad = []
-    for s in sorted(asm_dir.glob(f"*-hip-amdgcn-amd-amdhsa-{ARCH}.s")):
-        text = s.read_text()
-        hits = _PACKED_F32.findall(text)
-        if hits:
-            bad.append(f"{s.name}: {len(hits)} packed fp32 ops")
-    if bad:
-        raise RuntimeError("packed fp32 instructions in device code: " + "; ".join(bad))
-
-
 def build(verbose: bool = True) -> Path:
     """Compiles everything; returns the lib/ directory."""
     if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
@@ -87,7 +74,9 @@ def build(verbose: bool = True) -> Path:
         # --save-temps writes the gfx950 assembly next to the object (cwd)
         _run([HIPCC, *HIP_FLAGS, "--save-temps", "-c", CSRC / src, "-o", obj], cwd=obj_dir)
         objs.append(obj)
-    _check_no_packed_fp32(obj_dir)
+    import isa_check  # scans the gfx950 assembly written by --save-temps
+
+    isa_check.check_dir(obj_dir, ARCH)
 
     ic_obj = obj_dir / "ic.o"
     _run([CXX, *CXX_FLAGS, "-c", HOST / "ic.cpp", "-o", ic_obj])

@@ -8,23 +8,18 @@
 // Mapping (one workgroup = BLOCK threads x U targets):
 //   * target i's position and force accumulator live in VGPRs, U per thread
 //     (i = base + k * BLOCK + lane, so global loads/stores stay coalesced);
-//   * sources are streamed through LDS in tiles of TILE particles stored as
-//     three SoA float arrays; each lane reads 4 sources per coordinate with
-//     one ds_read_b128 at a wave-uniform address (an LDS broadcast: no bank
-//     conflicts);
-//   * the next tile is fetched from HBM into registers while the current
-//     one is consumed (register double-buffering, 2 barriers per tile);
+//   * sources stream through LDS tiles of TILE particles (nb_pairs.hpp:
+//     accumulate_sources), read back as wave-uniform b128 broadcasts;
 //   * optional source split (grid.y = S chunks) writes partial forces and a
 //     second kernel sums the S partials in chunk order and integrates.
-// Per interaction the pair loop issues 11 VALU ops + 1 v_rsq_f32:
-//   3 sub, 3 fma (|r|^2 + eps), 2 mul (d^3), rsq, 3 fma (F += r * inv).
-// No matrix cores: the pair loop is not a dense contraction.
+// The pair arithmetic (10 VALU + rsq + rcp per interaction) is in
+// nb_pairs.hpp.  No matrix cores: the pair loop is not a dense contraction.
 //
 // Build with -ffp-contract=off: the integrator must round each product and
-// sum separately, as written in the reference; the pair loop uses explicit
-// fmaf where a fused operation is intended.
+// sum separately, as written in the reference.
 
-#include "nb_internal.hpp"
+#include "nb_pairs.hpp"
+
 
 namespace nb {
 namespace {
@@ -40,192 +35,35 @@ struct FwdArgs {
   StepConsts c;
 };
 
-// Integrator of src/simulator.cu:214-228 in source order (no contraction).
-__device__ __forceinline__ void integrate(const FwdArgs &a, int i, float fx,
-                                          float fy, float fz) {
-  float vx = a.vx[i] * a.c.damping;
-  float vy = a.vy[i] * a.c.damping;
-  float vz = a.vz[i] * a.c.damping;
-  vx = vx + (fx * a.c.dt) * a.c.G;
-  vy = vy + (fy * a.c.dt) * a.c.G;
-  vz = vz + (fz * a.c.dt) * a.c.G;
-  a.vx[i] = vx;
-  a.vy[i] = vy;
-  a.vz[i] = vz;
-  a.nx[i] = a.px[i] + vx * a.c.dt;
-  a.ny[i] = a.py[i] + vy * a.c.dt;
-  a.nz[i] = a.pz[i] + vz * a.c.dt;
-}
-
-// Kahan step: s += term with running compensation c (term = r * inv fused
-// into the first subtraction).
-__device__ __forceinline__ void kahan_add(float r, float inv, float &s,
-                                          float &c) {
-  const float y = __builtin_fmaf(r, inv, -c);
-  const float t = s + y;
-  c = (t - s) - y;
-  s = t;
-}
-
-// One (i, j) interaction.  j and i are global indices (used by the masked
-// variants only).
-template <int MODE>
-__device__ __forceinline__ void pair(float xj, float yj, float zj, int j,
-                                     float xi, float yi, float zi, int i,
-                                     float eps, float &ax, float &ay,
-                                     float &az, float &cx, float &cy,
-                                     float &cz) {
-  const float rx = xj - xi;
-  const float ry = yj - yi;
-  const float rz = zj - zi;
-  float d;
-  if constexpr (MODE == MODE_PREDICATED) {
-    // dot(r, r) + distEps, as written (src/simulator.cu:201)
-    d = ((rx * rx + ry * ry) + rz * rz) + eps;
-  } else {
-    d = __builtin_fmaf(rx, rx, eps);
-    d = __builtin_fmaf(ry, ry, d);
-    d = __builtin_fmaf(rz, rz, d);
-  }
-  float inv = __builtin_amdgcn_rsqf((d * d) * d);
-  if constexpr (MODE == MODE_PLAIN_SKIP || MODE == MODE_KAHAN_SKIP) {
-    inv = (j == i) ? 0.0f : inv;
-  }
-  if constexpr (MODE == MODE_PLAIN || MODE == MODE_PLAIN_SKIP) {
-    ax = __builtin_fmaf(rx, inv, ax);
-    ay = __builtin_fmaf(ry, inv, ay);
-    az = __builtin_fmaf(rz, inv, az);
-  } else if constexpr (MODE == MODE_KAHAN || MODE == MODE_KAHAN_SKIP) {
-    kahan_add(rx, inv, ax, cx);
-    kahan_add(ry, inv, ay, cy);
-    kahan_add(rz, inv, az, cz);
-  } else {
-    // force += r * inv_dist_cube * (i == id)  (src/simulator.cu:209)
-    const float m = (j == i) ? 1.0f : 0.0f;
-    ax = ax + (rx * inv) * m;
-    ay = ay + (ry * inv) * m;
-    az = az + (rz * inv) * m;
-  }
-}
-
-template <int U, int MODE>
-struct Targets {
-  float x[U], y[U], z[U];
-  float ax[U], ay[U], az[U];
-  float cx[U], cy[U], cz[U];
-  int idx[U];
-
-  __device__ __forceinline__ void one(float xj, float yj, float zj, int j,
-                                      float eps) {
-#pragma unroll
-    for (int k = 0; k < U; ++k)
-      pair<MODE>(xj, yj, zj, j, x[k], y[k], z[k], idx[k], eps, ax[k], ay[k],
-                 az[k], cx[k], cy[k], cz[k]);
-  }
-
-  // Four consecutive sources j0..j0+3 (one b128 LDS read per coordinate).
-  __device__ __forceinline__ void four(float4 X, float4 Y, float4 Z, int j0,
-                                       float eps) {
-    one(X.x, Y.x, Z.x, j0 + 0, eps);
-    one(X.y, Y.y, Z.y, j0 + 1, eps);
-    one(X.z, Y.z, Z.z, j0 + 2, eps);
-    one(X.w, Y.w, Z.w, j0 + 3, eps);
-  }
-};
-
 template <int BLOCK, int U, int TILE, int MODE, bool SPLIT>
 __global__ __launch_bounds__(BLOCK) void fwd_kernel(FwdArgs a) {
-  static_assert(TILE % BLOCK == 0, "tile must be a multiple of the block");
-  static_assert(TILE % 4 == 0, "tile must hold whole float4 groups");
-  constexpr int L = TILE / BLOCK; // source loads per thread per coordinate
-
-  __shared__ float4 sx[TILE / 4];
-  __shared__ float4 sy[TILE / 4];
-  __shared__ float4 sz[TILE / 4];
-  float *fsx = reinterpret_cast<float *>(sx);
-  float *fsy = reinterpret_cast<float *>(sy);
-  float *fsz = reinterpret_cast<float *>(sz);
-
-  const int lane = threadIdx.x;
+  __shared__ SourceTile<TILE> sm;
   const int n = a.n;
-  const int base = blockIdx.x * (BLOCK * U);
-
   Targets<U, MODE> T;
-#pragma unroll
-  for (int k = 0; k < U; ++k) {
-    const int i = base + k * BLOCK + lane;
-    const int ic = i < n ? i : n - 1; // tail lanes compute on a valid copy
-    T.idx[k] = i;
-    T.x[k] = a.px[ic];
-    T.y[k] = a.py[ic];
-    T.z[k] = a.pz[ic];
-    T.ax[k] = T.ay[k] = T.az[k] = 0.0f;
-    T.cx[k] = T.cy[k] = T.cz[k] = 0.0f;
-  }
+  T.load(a.px, a.py, a.pz, blockIdx.x * (BLOCK * U), BLOCK, threadIdx.x, n);
 
   int jbeg = 0, jend = n;
   if constexpr (SPLIT) {
     jbeg = blockIdx.y * a.jchunk;
     jend = min(jbeg + a.jchunk, n);
   }
-  const float eps = a.c.eps;
+  accumulate_sources<BLOCK, TILE>(sm, T, a.px, a.py, a.pz, jbeg, jend,
+                                  a.c.eps);
 
-  float bx[L], by[L], bz[L];
-  auto fetch = [&](int j0) {
-#pragma unroll
-    for (int l = 0; l < L; ++l) {
-      const int j = j0 + l * BLOCK + lane;
-      const bool ok = j < jend;
-      bx[l] = ok ? a.px[j] : 0.0f;
-      by[l] = ok ? a.py[j] : 0.0f;
-      bz[l] = ok ? a.pz[j] : 0.0f;
-    }
-  };
-
-  if (jbeg < jend) fetch(jbeg);
-  for (int j0 = jbeg; j0 < jend; j0 += TILE) {
-    __syncthreads(); // the previous tile is no longer read
-#pragma unroll
-    for (int l = 0; l < L; ++l) {
-      fsx[l * BLOCK + lane] = bx[l];
-      fsy[l * BLOCK + lane] = by[l];
-      fsz[l * BLOCK + lane] = bz[l];
-    }
-    __syncthreads();
-    if (j0 + TILE < jend) fetch(j0 + TILE); // overlaps the pair loop below
-
-    const int cnt = min(TILE, jend - j0);
-    if (cnt == TILE) {
-#pragma unroll 2
-      for (int q = 0; q < TILE / 4; ++q)
-        T.four(sx[q], sy[q], sz[q], j0 + 4 * q, eps);
-    } else {
-      const int q4 = cnt >> 2;
-      for (int q = 0; q < q4; ++q)
-        T.four(sx[q], sy[q], sz[q], j0 + 4 * q, eps);
-      for (int jj = q4 * 4; jj < cnt; ++jj)
-        T.one(fsx[jj], fsy[jj], fsz[jj], j0 + jj, eps);
-    }
-  }
-
-  constexpr bool kKahan = (MODE == MODE_KAHAN || MODE == MODE_KAHAN_SKIP);
 #pragma unroll
   for (int k = 0; k < U; ++k) {
     const int i = T.idx[k];
     if (i >= n) continue;
-    float fx = T.ax[k], fy = T.ay[k], fz = T.az[k];
-    if constexpr (kKahan) {
-      fx = fx - T.cx[k];
-      fy = fy - T.cy[k];
-      fz = fz - T.cz[k];
-    }
+    float fx, fy, fz;
+    T.force(k, fx, fy, fz);
     if constexpr (SPLIT) {
       const size_t s = blockIdx.y;
       a.part[(s * 3 + 0) * n + i] = fx;
       a.part[(s * 3 + 1) * n + i] = fy;
       a.part[(s * 3 + 2) * n + i] = fz;
     } else {
-      integrate(a, i, fx, fy, fz);
+      integrate_ref(a.c, i, fx, fy, fz, a.px, a.py, a.pz, a.vx, a.vy, a.vz,
+                    a.nx, a.ny, a.nz);
     }
   }
 }
@@ -253,7 +91,8 @@ __global__ __launch_bounds__(256) void fwd_finalize(FwdArgs a) {
     }
     f[c] = s;
   }
-  integrate(a, i, f[0], f[1], f[2]);
+  integrate_ref(a.c, i, f[0], f[1], f[2], a.px, a.py, a.pz, a.vx, a.vy, a.vz,
+                a.nx, a.ny, a.nz);
 }
 
 using KernelFn = void (*)(FwdArgs);
@@ -285,18 +124,23 @@ const ShapeEntry kShapes[] = {
     {256, 1, 1024, shape_kernel<256, 1, 1024>},
     {256, 2, 1024, shape_kernel<256, 2, 1024>},
     {256, 4, 1024, shape_kernel<256, 4, 1024>},
-    {256, 8, 1024, shape_kernel<256, 8, 1024>},
-    {128, 4, 512, shape_kernel<128, 4, 512>},
+    {128, 1, 512, shape_kernel<128, 1, 512>},
+    {128, 2, 512, shape_kernel<128, 2, 512>},
+    {512, 1, 2048, shape_kernel<512, 1, 2048>},
     {512, 2, 2048, shape_kernel<512, 2, 2048>},
 };
 
-// Compiled launch shapes: {block, unroll, tile, split}.
+// Compiled launch shapes: {block, unroll, tile, split}.  Index 0 must stay
+// the unsplit 256 x 1 shape (a valid choice for every n).
 const FwdConfig kConfigs[] = {
-    {256, 1, 1024, 1}, {256, 2, 1024, 1}, {256, 2, 1024, 2},
-    {256, 4, 1024, 1}, {256, 4, 1024, 2}, {256, 4, 1024, 4},
-    {256, 8, 1024, 2}, {256, 8, 1024, 4}, {256, 8, 1024, 8},
-    {128, 4, 512, 1},  {128, 4, 512, 2},  {128, 4, 512, 4},
-    {512, 2, 2048, 1}, {512, 2, 2048, 2},
+    {256, 1, 1024, 1}, {256, 1, 1024, 2}, {256, 1, 1024, 4},
+    {256, 1, 1024, 8}, {256, 2, 1024, 1}, {256, 2, 1024, 2},
+    {256, 2, 1024, 4}, {256, 2, 1024, 8}, {256, 4, 1024, 1},
+    {256, 4, 1024, 2}, {256, 4, 1024, 4}, {256, 4, 1024, 8},
+    {128, 1, 512, 1},  {128, 1, 512, 2},  {128, 1, 512, 4},
+    {128, 2, 512, 2},  {128, 2, 512, 4},  {512, 1, 2048, 1},
+    {512, 1, 2048, 2}, {512, 2, 2048, 1}, {512, 2, 2048, 2},
+    {512, 2, 2048, 4},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
@@ -316,9 +160,9 @@ const FwdConfig &forward_config(int cfg) { return kConfigs[cfg]; }
 int forward_default_config(size_t n, int acc) {
   (void)acc;
   // Enough workgroups to fill 256 CUs several times over.
-  if (n >= (size_t)1 << 17) return 4; // 256 x 4, split 2
-  if (n >= (size_t)1 << 14) return 2; // 256 x 2, split 2
-  return 0;                           // 256 x 1
+  if (n >= (size_t)1 << 17) return 0; // 256 x 1
+  if (n >= (size_t)1 << 14) return 2; // 256 x 1, split 4
+  return 3;                           // 256 x 1, split 8
 }
 
 size_t forward_scratch_bytes(int cfg, size_t n) {

@@ -68,4 +68,24 @@ float *device_scratch(size_t bytes, int *status);
 // Pair mode for a parameter set.
 int pair_mode(int calc_method, int acc, float eps);
 
+// True when the self term of the pair sum is exactly zero (no j == i test
+// needed): dist_eps > 0 with a normal or infinite eps^3.
+bool self_term_vanishes(float eps);
+
+// Adjoint launch shapes (same fields as the forward ones).
+int adjoint_config_count();
+const FwdConfig &adjoint_config(int cfg);
+int adjoint_default_config(size_t n);
+size_t adjoint_scratch_bytes(int cfg, size_t n);
+
+// Adjoint of one step at positions pos: reads the cotangents of the step's
+// outputs (lxn, lvn), writes those of its inputs (lx, lv).  With fused set,
+// the same launch also applies the forward step (vel in/out, nxt written).
+int launch_adjoint(const StepConsts &c, bool skip, bool fused, int cfg,
+                   const float *const pos[3], const float *const lxn[3],
+                   const float *const lvn[3], float *const lx[3],
+                   float *const lv[3], float *const vel[3],
+                   float *const nxt[3], int n, float *scratch,
+                   hipStream_t stream);
+
 } // namespace nb

@@ -18,10 +18,13 @@ from typing import Sequence
 
 import numpy as np
 
+from .batch import Batch, batch_config_count, batch_config_info, plummer_batch_sizes
+from .adjoint import (adjoint_config_count, adjoint_config_info, adjoint_step,
+                      fwd_adjoint_step, trajectory)
 from ._lib import (DEFAULT_SEED, DRIVER_PATH, LIB_DIR, LIB_PATH, NB_ACC_KAHAN, NB_ACC_PLAIN,
                    NB_CALC_BRANCH, NB_CALC_PREDICATED, NB_ERR_INVALID_ARG, NB_ERR_NO_DEVICE,
                    NB_IC_LENGTH_F32, NB_IC_TRIG_DOUBLE, NB_OK, SIM_LIB_PATH, NbError, NbParams,
-                   check, default_params, load)
+                   NbVec3, check, default_params, load)
 
 __all__ = [
     "DEFAULT_SEED", "DRIVER_PATH", "LIB_DIR", "LIB_PATH", "SIM_LIB_PATH", "NB_ACC_KAHAN",
@@ -29,7 +32,9 @@ __all__ = [
     "NB_ERR_NO_DEVICE", "NB_IC_LENGTH_F32", "NB_IC_TRIG_DOUBLE", "NB_OK", "NbError", "NbParams",
     "Simulator", "check", "default_params", "device_count", "device_name", "disk_ic",
     "disk_gen_ic", "forward_config_count", "forward_config_info", "forward_step", "load",
-    "plummer_ic",
+    "plummer_ic", "adjoint_config_count", "adjoint_config_info", "adjoint_step",
+    "fwd_adjoint_step", "trajectory", "NbVec3", "Batch", "batch_config_count",
+    "batch_config_info", "plummer_batch_sizes",
 ]
 
 

@@ -50,6 +50,12 @@ class NbParams(ctypes.Structure):
     ]
 
 
+class NbVec3(ctypes.Structure):
+    """nb_vec3: three device arrays of one SoA vector field."""
+
+    _fields_ = [("x", c_void_p), ("y", c_void_p), ("z", c_void_p)]
+
+
 class NbError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"nbody_amd error {code}: {msg}")
@@ -110,6 +116,33 @@ def load() -> ctypes.CDLL:
     _sig(lib, "nb_sim_destroy", c_int, [vp])
     _sig(lib, "nb_forward_step_dev", c_int,
          [POINTER(NbParams)] + [vp] * 9 + [c_size_t, c_int, c_int, vp])
+    V3 = POINTER(NbVec3)
+    _sig(lib, "nb_adjoint_step_dev", c_int,
+         [POINTER(NbParams), V3, V3, V3, V3, V3, c_size_t, c_int, vp])
+    _sig(lib, "nb_fwd_adjoint_step_dev", c_int,
+         [POINTER(NbParams), V3, V3, V3, V3, V3, V3, V3, c_size_t, c_int, vp])
+    _sig(lib, "nb_adjoint_config_count", c_int, [])
+    _sig(lib, "nb_adjoint_config_info", c_int, [c_int] + [POINTER(c_int)] * 4)
+    _sig(lib, "nb_traj_checkpoint_floats", c_size_t, [c_size_t, c_int, c_int])
+    _sig(lib, "nb_traj_workspace_floats", c_size_t, [c_size_t, c_int])
+    _sig(lib, "nb_traj_forward_dev", c_int,
+         [POINTER(NbParams), vp, vp, c_int, c_int, vp, vp, c_size_t, vp])
+    _sig(lib, "nb_traj_backward_dev", c_int,
+         [POINTER(NbParams), vp, vp, vp, c_int, c_int, vp, c_size_t, vp])
+    _sig(lib, "nb_batch_create", c_int,
+         [c_int, POINTER(NbParams)] + [vp] * 6 + [POINTER(c_void_p)])
+    for name in ("nb_batch_step", "nb_batch_step_direct", "nb_batch_step_separate",
+                 "nb_batch_set_config"):
+        _sig(lib, name, c_int, [vp, c_int])
+    _sig(lib, "nb_batch_autotune", c_int, [vp, c_int, POINTER(c_int)])
+    _sig(lib, "nb_batch_config", c_int, [vp])
+    _sig(lib, "nb_batch_config_count", c_int, [])
+    _sig(lib, "nb_batch_config_info", c_int, [c_int] + [POINTER(c_int)] * 3)
+    _sig(lib, "nb_batch_last_ms", c_float, [vp])
+    _sig(lib, "nb_batch_num_particles", c_size_t, [vp])
+    _sig(lib, "nb_batch_read_pos", c_int, [vp, vp, vp, vp])
+    _sig(lib, "nb_batch_read_vel", c_int, [vp, vp, vp, vp])
+    _sig(lib, "nb_batch_destroy", c_int, [vp])
     _sig(lib, "nb_forward_config_count", c_int, [])
     _sig(lib, "nb_forward_config_info", c_int, [c_int] + [POINTER(c_int)] * 4)
     _lib = lib

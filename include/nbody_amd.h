@@ -168,6 +168,92 @@ int nb_forward_config_count(void);
 int nb_forward_config_info(int cfg, int *block, int *unroll, int *tile,
                            int *split);
 
+
+/* ---- adjoint (extension; no reference counterpart) ---------------------- */
+/* Three device arrays of one SoA vector field. */
+typedef struct nb_vec3 {
+  float *x;
+  float *y;
+  float *z;
+} nb_vec3;
+
+/* Reverse-mode derivative of one step (x, v) -> (x', v') of
+ * particle_interaction's integrator (src/simulator.cu:214-228):
+ *   g = lam_v_next + dt * lam_x_next,  lam_v = damping * g,
+ *   lam_x = lam_x_next + dt * G * sum_{j != k} M_kj (g_j - g_k),
+ *   M = K (I - 3 r r^T / d),  d = |r|^2 + eps,  K = d^-3/2,
+ * evaluated at positions pos (x_n) in one pass over all pairs.  BRANCH only.
+ * lam_x / lam_v must not alias the inputs.  cfg: -1 = library's shape. */
+int nb_adjoint_step_dev(const nb_params *p, const nb_vec3 *pos,
+                        const nb_vec3 *lam_x_next, const nb_vec3 *lam_v_next,
+                        const nb_vec3 *lam_x, const nb_vec3 *lam_v, size_t n,
+                        int cfg, void *stream);
+
+/* The forward step and the adjoint step at the same positions in one
+ * launch: vel is updated in place and pos_next written (as
+ * nb_forward_step_dev), and the cotangents propagated (as
+ * nb_adjoint_step_dev).  The pair quantities are computed once. */
+int nb_fwd_adjoint_step_dev(const nb_params *p, const nb_vec3 *pos,
+                            const nb_vec3 *vel, const nb_vec3 *pos_next,
+                            const nb_vec3 *lam_x_next,
+                            const nb_vec3 *lam_v_next, const nb_vec3 *lam_x,
+                            const nb_vec3 *lam_v, size_t n, int cfg,
+                            void *stream);
+
+int nb_adjoint_config_count(void);
+int nb_adjoint_config_info(int cfg, int *block, int *unroll, int *tile,
+                           int *split);
+
+/* ---- differentiable trajectory with checkpoints (extension) ------------- */
+/* States are [6][n] float blocks: x, y, z, vx, vy, vz (n floats each).
+ * The forward pass runs `steps` steps from state0, stores the state every
+ * `every` steps (checkpoint slot s = state at step s * every) and writes
+ * the final state.  The backward pass takes the cotangent of the final state
+ * ([6][n]: lam_x then lam_v), recomputes each segment's positions from its
+ * checkpoint and runs the adjoint steps in reverse, writing the cotangent of
+ * state0.  Both are stream-ordered and allocate nothing: the caller passes
+ * checkpoint storage and a workspace of the sizes below (in floats). */
+size_t nb_traj_checkpoint_floats(size_t n, int steps, int every);
+size_t nb_traj_workspace_floats(size_t n, int every);
+int nb_traj_forward_dev(const nb_params *p, const float *state0,
+                        float *state_final, int steps, int every,
+                        float *checkpoints, float *workspace, size_t n,
+                        void *stream);
+int nb_traj_backward_dev(const nb_params *p, const float *checkpoints,
+                         const float *lam_final, float *lam0, int steps,
+                         int every, float *workspace, size_t n, void *stream);
+
+/* ---- batched independent systems (extension) ---------------------------- */
+/* Many independent systems advanced together.  System s has
+ * params[s].num_particles particles (BRANCH; its own G, dt, damping,
+ * dist_eps); the host arrays hold all systems back to back.  One step is
+ * one launch of a persistent kernel that drains a device work queue of
+ * (system, target block) items, longest first; steps are replayed from a
+ * captured two-step hipGraph. */
+typedef struct nb_batch nb_batch;
+int nb_batch_create(int num_systems, const nb_params *params, const float *px,
+                    const float *py, const float *pz, const float *vx,
+                    const float *vy, const float *vz, nb_batch **out);
+/* `steps` steps through the captured graph; synchronises and records the
+ * device time in nb_batch_last_ms. */
+int nb_batch_step(nb_batch *b, int steps);
+/* The same steps as direct launches (no graph). */
+int nb_batch_step_direct(nb_batch *b, int steps);
+/* Baseline: one nb_forward_step-style launch per system per step. */
+int nb_batch_step_separate(nb_batch *b, int steps);
+/* Times every compiled launch shape on this batch (trial_steps each, the
+ * state restored after each trial) and keeps the fastest. */
+int nb_batch_autotune(nb_batch *b, int trial_steps, int *best_cfg);
+int nb_batch_set_config(nb_batch *b, int cfg);
+int nb_batch_config(const nb_batch *b);
+int nb_batch_config_count(void);
+int nb_batch_config_info(int cfg, int *block, int *unroll, int *tile);
+float nb_batch_last_ms(const nb_batch *b);
+size_t nb_batch_num_particles(const nb_batch *b);
+int nb_batch_read_pos(nb_batch *b, float *x, float *y, float *z);
+int nb_batch_read_vel(nb_batch *b, float *x, float *y, float *z);
+int nb_batch_destroy(nb_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

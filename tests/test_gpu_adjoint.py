@@ -1,0 +1,204 @@
+"""GPU adjoint kernels and the checkpointed trajectory gradient, against the
+fp64 CPU adjoint of the oracle (itself pinned by central differences in
+test_oracle.py) and against central finite differences of an fp64 loss.
+
+Gradient error metric: ||g_gpu - g_fp64|| / ||g_fp64|| (2-norm over all
+particles and components); target <= 1e-3.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def to_dev(torch, arrs, gpu):
+    return torch.from_numpy(np.ascontiguousarray(np.stack(arrs), np.float32)).to(gpu)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_adjoint_pair_sum_vs_fp64(nb, orc, gpu):
+    """With dt = G = 1 and lam_x' = 0 the kernel's lam_x is the pair sum
+    J^T g itself; compare it per particle with the fp64 sum."""
+    import torch
+
+    n = 3000
+    pos, _ = nb.plummer_ic(n, seed=21, g_mass=1.0)
+    rng = np.random.default_rng(2)
+    g = rng.normal(size=(3, n)).astype(np.float32)
+    params = nb.default_params(num_particles=n, dt=1.0, G=1.0, damping=0.5)
+    P = to_dev(torch, pos, gpu)
+    lxn = torch.zeros(3, n, device=gpu)
+    lvn = torch.from_numpy(g).to(gpu)
+    lx = torch.empty(3, n, device=gpu)
+    lv = torch.empty(3, n, device=gpu)
+    nb.adjoint_step(params, P, lxn, lvn, lx, lv)
+    torch.cuda.synchronize()
+    p = orc.params(dt=1.0, G=1.0, damping=0.5)
+    ref_lx, ref_lv = orc.adjoint_step_f64(p, pos, np.zeros((3, n)), g.astype(np.float64))
+    got = lx.cpu().numpy().astype(np.float64)
+    ref = np.stack(ref_lx)
+    e = np.abs(got - ref).max(0) / np.abs(ref).max()
+    print(f"J^T g vs fp64: max {e.max():.2e} normwise {rel(got, ref):.2e}")
+    assert rel(got, ref) < 1e-5 and e.max() < 1e-4
+    assert np.array_equal(lv.cpu().numpy(), np.float32(0.5) * g)
+
+
+def test_adjoint_step_default_params(nb, orc, gpu):
+    import torch
+
+    n = 4096
+    pos, _ = nb.disk_ic(n)
+    rng = np.random.default_rng(3)
+    lxn = rng.normal(size=(3, n)).astype(np.float32)
+    lvn = rng.normal(size=(3, n)).astype(np.float32)
+    params = nb.default_params(num_particles=n)
+    outs = [torch.empty(3, n, device=gpu) for _ in range(2)]
+    nb.adjoint_step(params, to_dev(torch, pos, gpu), torch.from_numpy(lxn).to(gpu),
+                    torch.from_numpy(lvn).to(gpu), *outs)
+    torch.cuda.synchronize()
+    ref_lx, ref_lv = orc.adjoint_step_f64(orc.params(), pos, lxn, lvn)
+    # compare the pair-sum part: (lam_x - lam_x') / (dt G)
+    c = float(np.float32(0.005)) * 2.0
+    got = (outs[0].cpu().numpy().astype(np.float64) - lxn) / c
+    ref = (np.stack(ref_lx) - lxn) / c
+    print(f"default params: pair part normwise {rel(got, ref):.2e}")
+    assert rel(got, ref) < 1e-4
+    assert rel(outs[1].cpu().numpy(), np.stack(ref_lv)) < 1e-6
+
+
+@pytest.mark.parametrize("n", [1, 2, 257, 1500])
+def test_fused_step_equals_separate(nb, gpu, n):
+    """Forward part bit-equal to the forward kernel, adjoint part bit-equal
+    to the adjoint kernel (same launch shape, same source order)."""
+    import torch
+
+    pos, vel = nb.plummer_ic(n, seed=n, g_mass=1.0)
+    params = nb.default_params(num_particles=n, G=1.0 / n)
+    rng = np.random.default_rng(n)
+    lxn = torch.from_numpy(rng.normal(size=(3, n)).astype(np.float32)).to(gpu)
+    lvn = torch.from_numpy(rng.normal(size=(3, n)).astype(np.float32)).to(gpu)
+    P = to_dev(torch, pos, gpu)
+    V1 = to_dev(torch, vel, gpu)
+    V2 = V1.clone()
+    N1 = torch.empty_like(P)
+    N2 = torch.empty_like(P)
+    lx1, lv1, lx2, lv2 = [torch.empty(3, n, device=gpu) for _ in range(4)]
+    nb.forward_step(params, list(P), list(V1), list(N1), cfg=0)
+    nb.adjoint_step(params, P, lxn, lvn, lx1, lv1, cfg=0)
+    nb.fwd_adjoint_step(params, P, V2, N2, lxn, lvn, lx2, lv2, cfg=0)
+    torch.cuda.synchronize()
+    for a, b in ((V1, V2), (N1, N2), (lx1, lx2), (lv1, lv2)):
+        assert torch.equal(a, b)
+
+
+def test_every_adjoint_shape(nb, gpu):
+    import torch
+
+    n = 5000
+    pos, _ = nb.disk_ic(n)
+    rng = np.random.default_rng(5)
+    lxn = torch.from_numpy(rng.normal(size=(3, n)).astype(np.float32)).to(gpu)
+    lvn = torch.from_numpy(rng.normal(size=(3, n)).astype(np.float32)).to(gpu)
+    params = nb.default_params(num_particles=n)
+    P = to_dev(torch, pos, gpu)
+    ref = None
+    for cfg in range(nb.adjoint_config_count()):
+        lx, lv = torch.empty(3, n, device=gpu), torch.empty(3, n, device=gpu)
+        nb.adjoint_step(params, P, lxn, lvn, lx, lv, cfg=cfg)
+        torch.cuda.synchronize()
+        got = (lx - lxn).double().cpu().numpy()
+        if ref is None:
+            ref = got
+        assert rel(got, ref) < 1e-5, (cfg, nb.adjoint_config_info(cfg))
+
+
+def disk_problem(nb, orc, torch, gpu, n, steps, every):
+    (pos, vel) = nb.disk_ic(n)
+    x0 = to_dev(torch, pos, gpu)
+    v0 = to_dev(torch, vel, gpu)
+    params = nb.default_params(num_particles=n)
+    rng = np.random.default_rng(7)
+    pert = torch.from_numpy(rng.normal(scale=1e-3, size=(3, n)).astype(np.float32)).to(gpu)
+    with torch.no_grad():
+        target, _ = nb.trajectory(x0 + pert, v0, params, steps, every)
+    return x0, v0, params, target
+
+
+def test_trajectory_gradient_vs_fp64(nb, orc, gpu):
+    import torch
+
+    n, T, K = 2048, 40, 10
+    x0, v0, params, target = disk_problem(nb, orc, torch, gpu, n, T, K)
+    x = x0.clone().requires_grad_(True)
+    v = v0.clone().requires_grad_(True)
+    xT, vT = nb.trajectory(x, v, params, T, K)
+    loss = ((xT.double() - target.double()) ** 2).sum(0).mean()
+    loss.backward()
+    gx = x.grad.double().cpu().numpy()
+    gv = v.grad.double().cpu().numpy()
+    L64, gx64, gv64 = orc.traj_grad_f64(orc.params(), x0.cpu().numpy(), v0.cpu().numpy(),
+                                        target.double().cpu().numpy(), T)
+    ex, ev = rel(gx, gx64), rel(gv, gv64)
+    print(f"N={n} T={T} K={K}: loss {loss.item():.6e} (fp64 {L64:.6e}); "
+          f"grad rel err x {ex:.2e} v {ev:.2e}")
+    assert abs(loss.item() - L64) <= 1e-3 * L64
+    assert ex <= 1e-3 and ev <= 1e-3
+
+
+def test_checkpoint_interval_does_not_change_gradient(nb, orc, gpu):
+    import torch
+
+    n, T = 1024, 24
+    x0, v0, params, target = disk_problem(nb, orc, torch, gpu, n, T, 5)
+    grads = []
+    for K in (1, 5, 7, 24, 100):
+        x = x0.clone().requires_grad_(True)
+        v = v0.clone().requires_grad_(True)
+        xT, _ = nb.trajectory(x, v, params, T, K)
+        ((xT - target) ** 2).sum(0).mean().backward()
+        grads.append((x.grad.clone(), v.grad.clone()))
+    for gx, gv in grads[1:]:  # replayed states are bit-identical
+        assert torch.equal(gx, grads[0][0]) and torch.equal(gv, grads[0][1])
+
+
+def test_trajectory_central_differences(nb, orc, gpu):
+    """Adjoint gradient vs central differences (h = 1e-3) of the fp64 loss
+    on 32 random coordinates of x0 and v0."""
+    import torch
+
+    n, T, K = 1024, 30, 10
+    x0, v0, params, target = disk_problem(nb, orc, torch, gpu, n, T, K)
+    x = x0.clone().requires_grad_(True)
+    v = v0.clone().requires_grad_(True)
+    xT, _ = nb.trajectory(x, v, params, T, K)
+    ((xT.double() - target.double()) ** 2).sum(0).mean().backward()
+    gx = x.grad.double().cpu().numpy()
+    gv = v.grad.double().cpu().numpy()
+    X0 = x0.double().cpu().numpy()
+    V0 = v0.double().cpu().numpy()
+    tgt = target.double().cpu().numpy()
+    p = orc.params()
+    rng = np.random.default_rng(11)
+    h = 1e-3
+    fd, an = [], []
+    for k in range(32):
+        c, i = int(rng.integers(3)), int(rng.integers(n))
+        which = k % 2
+        Xp, Xm, Vp, Vm = X0.copy(), X0.copy(), V0.copy(), V0.copy()
+        if which == 0:
+            Xp[c, i] += h
+            Xm[c, i] -= h
+        else:
+            Vp[c, i] += h
+            Vm[c, i] -= h
+        Lp, _, _ = orc.traj_grad_f64(p, Xp, Vp, tgt, T, grad=False)
+        Lm, _, _ = orc.traj_grad_f64(p, Xm, Vm, tgt, T, grad=False)
+        fd.append((Lp - Lm) / (2 * h))
+        an.append(gx[c, i] if which == 0 else gv[c, i])
+    fd, an = np.array(fd), np.array(an)
+    err = rel(an, fd)
+    print(f"central differences on 32 coordinates: rel err {err:.2e}")
+    assert err <= 1e-2
