@@ -12,6 +12,16 @@ the particles resident in HBM.  `roofline` prices the pair kernel against
 MI355X's fp32 vector peak (157.3 TFLOP/s) using HIP events on the launch
 stream.  `cpu_baseline` times the fp32 restatement of the reference kernel
 (oracle, OpenMP) on a bounded sample of the same step on the host cores.
+
+Also in the line (same run, same N):
+  fwd_adjoint  the fused forward+adjoint step (one launch, N^2 pairs) and the
+               separate adjoint step, as Ginteractions/s and as a multiple of
+               the forward step time;
+  trajectory   a checkpointed differentiable trajectory (forward pass +
+               backward pass with segment replay) per step;
+  batched      64 independent Plummer systems (N_s uniform on
+               {1024, 1280, ..., 32768}): the autotuned persistent kernel
+               replayed from a hipGraph vs one plain launch per system.
 """
 from __future__ import annotations
 
@@ -40,7 +50,118 @@ def parse_args():
     ap.add_argument("--cfg", type=int, default=-1, help="forward launch shape (-1: default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the fwd+adjoint, trajectory and batched sections")
+    ap.add_argument("--batch-steps", type=int, default=200)
     return ap.parse_args()
+
+
+def _events(torch, fn, reps: int) -> float:
+    """Mean ms per call of fn() on the current stream (HIP events)."""
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def fwd_adjoint_section(nb, torch, pos, vel, n: int, fwd_ms: float) -> dict:
+    import numpy as np
+
+    dev = torch.device("cuda:0")
+    params = nb.default_params(num_particles=n)
+    P = torch.from_numpy(np.stack(pos)).to(dev)
+    V = torch.from_numpy(np.stack(vel)).to(dev)
+    N = torch.empty_like(P)
+    rng = np.random.default_rng(0)
+    lxn = torch.from_numpy(rng.normal(size=(3, n)).astype(np.float32)).to(dev)
+    lvn = torch.from_numpy(rng.normal(size=(3, n)).astype(np.float32)).to(dev)
+    lx, lv = torch.empty_like(lxn), torch.empty_like(lxn)
+    reps = 5
+    adj_ms = _events(torch, lambda: nb.adjoint_step(params, P, lxn, lvn, lx, lv), reps)
+    fused_ms = _events(torch, lambda: nb.fwd_adjoint_step(params, P, V, N, lxn, lvn, lx, lv),
+                       reps)
+    inter = float(n) * n
+    return {
+        "value": round(inter / (fused_ms * 1e-3) / 1e9, 2),
+        "unit": "Ginteractions/s",
+        "ms_per_step": round(fused_ms, 4),
+        "ratio_to_fwd": round(fused_ms / fwd_ms, 3),
+        "target_ratio": 3.5,
+        "adjoint_only_ms": round(adj_ms, 4),
+        "fwd_plus_adjoint_ratio": round((fwd_ms + adj_ms) / fwd_ms, 3),
+        "note": "fused = one launch computing the forward step and the adjoint at the same "
+                "positions (pair terms shared); ratios are against the plain forward step",
+    }
+
+
+def trajectory_section(nb, torch, pos, vel, n: int, fwd_ms: float, steps=20, every=10) -> dict:
+    import numpy as np
+
+    dev = torch.device("cuda:0")
+    params = nb.default_params(num_particles=n)
+    x0 = torch.from_numpy(np.stack(pos)).to(dev).requires_grad_(True)
+    v0 = torch.from_numpy(np.stack(vel)).to(dev).requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xT, vT = nb.trajectory(x0, v0, params, steps, every)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss = (xT * xT).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    fwd_pass = (t1 - t0) * 1e3 / steps
+    bwd_pass = (t2 - t1) * 1e3 / steps
+    return {
+        "steps": steps,
+        "checkpoint_every": every,
+        "forward_pass_ms_per_step": round(fwd_pass, 3),
+        "backward_pass_ms_per_step": round(bwd_pass, 3),
+        "total_ratio_to_fwd": round((fwd_pass + bwd_pass) / fwd_ms, 3),
+        "note": "backward = segment replay from checkpoints (forward kernel) + adjoint kernel",
+    }
+
+
+def batched_section(nb, steps: int) -> dict:
+    import numpy as np
+
+    sizes = nb.plummer_batch_sizes(64, seed=0)
+    params, pos, vel = [], [[], [], []], [[], [], []]
+    for k, m in enumerate(sizes):
+        p, v = nb.plummer_ic(m, seed=1000 + k, a=1.0, g_mass=1.0)
+        params.append(nb.default_params(num_particles=m, G=1.0 / m))
+        for c in range(3):
+            pos[c].append(p[c])
+            vel[c].append(v[c])
+    b = nb.Batch(params, tuple(np.concatenate(a) for a in pos),
+                 tuple(np.concatenate(a) for a in vel))
+    best = b.autotune(2)
+    b.step(2)
+    graph_ms = b.step(steps) / steps
+    b.step_separate(2)
+    sep_ms = b.step_separate(steps) / steps
+    b.close()
+    inter = float(sum(m * m for m in sizes))
+    return {
+        "systems": len(sizes),
+        "particles": int(sum(sizes)),
+        "interactions_per_step": inter,
+        "steps": steps,
+        "value": round(inter / (graph_ms * 1e-3) / 1e9, 2),
+        "unit": "Ginteractions/s",
+        "ms_per_step": round(graph_ms, 4),
+        "separate_launches_ms_per_step": round(sep_ms, 4),
+        "separate_launches_value": round(inter / (sep_ms * 1e-3) / 1e9, 2),
+        "speedup_vs_separate": round(sep_ms / graph_ms, 3),
+        "autotuned_launch": nb.batch_config_info(best),
+        "ics": "Plummer a=1, G*M=1 per system (G = 1/N_s), seeds 1000+s; sizes seed 0",
+    }
 
 
 def traffic_from_profile(n: int):
@@ -147,6 +268,10 @@ def main() -> int:
             "kernel_ms_per_launch": round(per_launch_ms, 4),
         },
     }
+    if not args.no_extras:
+        line["fwd_adjoint"] = fwd_adjoint_section(nb, torch, pos, vel, n, per_launch_ms)
+        line["trajectory"] = trajectory_section(nb, torch, pos, vel, n, per_launch_ms)
+        line["batched"] = batched_section(nb, args.batch_steps)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(pos, vel, n, args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -10,9 +10,9 @@
 //   lx_k = lx'_k + c sum_{j != k} M_kj (g_j - g_k),
 //   M(r) = df/dr = K (I - 3 r r^T / d),  d = |r|^2 + eps   (symmetric, even).
 // The pair term is evaluated as K (h + t3 r) with h = g_j - g_k and
-// t3 = -3 (r . h) / d: per interaction 21 VALU ops + rsq(d) + rcp(d), the
+// t3 = -3 (r . h) / d: per interaction 22 VALU ops + rsq(d^3) + rcp(d), the
 // pair quantities r, d, K recomputed in registers (no stored forward state
-// beyond the positions).
+// beyond the positions).  K is formed exactly as in the forward kernel.
 //
 // The fused variant also accumulates F(x) = sum K r in the same pass (3 more
 // FMAs) and applies the forward step, so one launch advances the state and
@@ -60,9 +60,10 @@ struct AdjTargets {
       float d = __builtin_fmaf(rx, rx, eps);
       d = __builtin_fmaf(ry, ry, d);
       d = __builtin_fmaf(rz, rz, d);
-      const float s = __builtin_amdgcn_rsqf(d);
+      // K exactly as the forward kernel forms it (rsqrt(d^3)), so the fused
+      // variant's forward outputs are bit-identical to the forward kernel's
+      float K = __builtin_amdgcn_rsqf((d * d) * d);
       const float q = __builtin_amdgcn_rcpf(d);
-      float K = s * q;
       if constexpr (SKIP) K = (j == idx[k]) ? 0.0f : K;
       const float hx = gjx - gx[k];
       const float hy = gjy - gy[k];

@@ -18,6 +18,7 @@ from typing import Sequence
 
 import numpy as np
 
+from .diag import energy_momentum, forces_f64
 from .batch import Batch, batch_config_count, batch_config_info, plummer_batch_sizes
 from .adjoint import (adjoint_config_count, adjoint_config_info, adjoint_step,
                       fwd_adjoint_step, trajectory)
@@ -34,7 +35,7 @@ __all__ = [
     "disk_gen_ic", "forward_config_count", "forward_config_info", "forward_step", "load",
     "plummer_ic", "adjoint_config_count", "adjoint_config_info", "adjoint_step",
     "fwd_adjoint_step", "trajectory", "NbVec3", "Batch", "batch_config_count",
-    "batch_config_info", "plummer_batch_sizes",
+    "batch_config_info", "plummer_batch_sizes", "energy_momentum", "forces_f64",
 ]
 
 

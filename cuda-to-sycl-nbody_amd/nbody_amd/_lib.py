@@ -143,6 +143,10 @@ def load() -> ctypes.CDLL:
     _sig(lib, "nb_batch_read_pos", c_int, [vp, vp, vp, vp])
     _sig(lib, "nb_batch_read_vel", c_int, [vp, vp, vp, vp])
     _sig(lib, "nb_batch_destroy", c_int, [vp])
+    _sig(lib, "nb_forces_f64_dev", c_int,
+         [POINTER(NbParams), V3, vp, c_size_t, vp, vp, vp, c_size_t, vp])
+    _sig(lib, "nb_energy_momentum_dev", c_int,
+         [POINTER(NbParams), V3, V3, c_size_t, vp, vp])
     _sig(lib, "nb_forward_config_count", c_int, [])
     _sig(lib, "nb_forward_config_info", c_int, [c_int] + [POINTER(c_int)] * 4)
     _lib = lib

@@ -254,6 +254,22 @@ int nb_batch_read_pos(nb_batch *b, float *x, float *y, float *z);
 int nb_batch_read_vel(nb_batch *b, float *x, float *y, float *z);
 int nb_batch_destroy(nb_batch *b);
 
+/* ---- fp64 diagnostics (extension) --------------------------------------- */
+/* fp64 oracle force (BRANCH semantics, the step's eps) on the targets
+ * idx[0..m) (device int32 indices; an index outside [0, n) yields NaN) from
+ * all n float sources: F_i = sum_{j != i} r (|r|^2 + eps)^{-3/2}, r formed in
+ * double.  fx, fy, fz: device double arrays of m values. */
+int nb_forces_f64_dev(const nb_params *p, const nb_vec3 *pos,
+                      const int32_t *idx, size_t m, double *fx, double *fy,
+                      double *fz, size_t n, void *stream);
+
+/* fp64 totals into the device array out[5]: kinetic sum |v|^2 / 2,
+ * potential -G sum_{i<j} (|r|^2 + eps)^{-1/2} (whose negative gradient is
+ * the step's force), momentum sum v (unit masses).  Deterministic order. */
+int nb_energy_momentum_dev(const nb_params *p, const nb_vec3 *pos,
+                           const nb_vec3 *vel, size_t n, double *out,
+                           void *stream);
+
 #ifdef __cplusplus
 }
 #endif
