@@ -103,6 +103,9 @@ struct nb_sim {
   int acc = NB_ACC_PLAIN;
   int cfg = 0;
   float last_ms = 0.0f, last_kernel_ms = 0.0f;
+  float *staging = nullptr; // pinned host copy of pos and vel (6 arrays)
+  hipEvent_t ev_stage = nullptr;
+  bool staged = false;
 };
 
 namespace {
@@ -137,6 +140,8 @@ void destroy_sim(nb_sim *s) {
   if (s->scratch) (void)hipFree(s->scratch);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->ev_stage) (void)hipEventDestroy(s->ev_stage);
+  if (s->staging) (void)hipHostFree(s->staging);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -316,6 +321,69 @@ static int read3(nb_sim *s, float *const src[3], float *x, float *y,
                        "device-to-host copy");
     if (st) return st;
   }
+  return hip_check(hipStreamSynchronize(s->stream), "stream synchronize");
+}
+
+int nb_sim_stage_readback(nb_sim *s) {
+  if (!s) return set_error(NB_ERR_INVALID_ARG, "sim is NULL");
+  if (s->n == 0) {
+    s->staged = true;
+    return NB_OK;
+  }
+  int st = NB_OK;
+  if (!s->staging) {
+    void *p = nullptr;
+    if ((st = hip_check(hipHostMalloc(&p, sizeof(float) * 6 * s->stride,
+                                      hipHostMallocDefault),
+                        "pinned hipHostMalloc")))
+      return st;
+    s->staging = static_cast<float *>(p);
+  }
+  if (!s->ev_stage &&
+      (st = hip_check(hipEventCreateWithFlags(&s->ev_stage,
+                                              hipEventDisableTiming),
+                      "hipEventCreate")))
+    return st;
+  float *const src[6] = {s->pos[0], s->pos[1], s->pos[2],
+                         s->vel[0], s->vel[1], s->vel[2]};
+  for (int c = 0; c < 6; ++c)
+    if ((st = hip_check(hipMemcpyAsync(s->staging + c * s->stride, src[c],
+                                       sizeof(float) * s->n,
+                                       hipMemcpyDeviceToHost, s->stream),
+                        "staged device-to-host copy")))
+      return st;
+  if ((st = hip_check(hipEventRecord(s->ev_stage, s->stream), "hipEventRecord")))
+    return st;
+  s->staged = true;
+  return NB_OK;
+}
+
+int nb_sim_staged(nb_sim *s, const float **arrays) {
+  if (!s || !arrays) return set_error(NB_ERR_INVALID_ARG, "NULL argument");
+  if (!s->staged)
+    return set_error(NB_ERR_INVALID_ARG, "nb_sim_stage_readback was not called");
+  for (int c = 0; c < 6; ++c)
+    arrays[c] = s->staging ? s->staging + c * s->stride : nullptr;
+  if (s->n == 0) return NB_OK;
+  return hip_check(hipEventSynchronize(s->ev_stage), "hipEventSynchronize");
+}
+
+int nb_sim_read_vec4(nb_sim *s, int which, float w, float *out) {
+  if (!s) return set_error(NB_ERR_INVALID_ARG, "sim is NULL");
+  if (which != 0 && which != 1)
+    return set_error(NB_ERR_INVALID_ARG, "which must be 0 (pos) or 1 (vel)");
+  if (s->n == 0) return NB_OK;
+  if (!out) return set_error(NB_ERR_INVALID_ARG, "output is NULL");
+  int st = NB_OK;
+  float *buf = device_scratch(sizeof(float) * 4 * s->n, &st);
+  if (st) return st;
+  float *const *src = which == 0 ? s->pos : s->vel;
+  if ((st = launch_pack_vec4(src[0], src[1], src[2], w, buf, (int)s->n,
+                             s->stream)) ||
+      (st = hip_check(hipMemcpyAsync(out, buf, sizeof(float) * 4 * s->n,
+                                     hipMemcpyDeviceToHost, s->stream),
+                      "device-to-host copy")))
+    return st;
   return hip_check(hipStreamSynchronize(s->stream), "stream synchronize");
 }
 

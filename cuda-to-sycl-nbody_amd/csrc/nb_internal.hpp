@@ -6,6 +6,7 @@
 
 #include "nbody_amd.h"
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -64,6 +65,10 @@ size_t forward_scratch_bytes(int cfg, size_t n);
 // Per-device scratch owned by the library (grown on demand, never shrunk).
 // Calls on different streams of one device share it.
 float *device_scratch(size_t bytes, int *status);
+
+// SoA -> AoS vec4 pack on the device (out: 4 n floats, 16 B aligned).
+int launch_pack_vec4(const float *x, const float *y, const float *z, float w,
+                     float *out, int n, hipStream_t stream);
 
 // Pair mode for a parameter set.
 int pair_mode(int calc_method, int acc, float eps);

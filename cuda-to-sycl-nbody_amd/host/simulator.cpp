@@ -7,6 +7,7 @@
 
 #include "simulator.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 #include <string>
@@ -103,10 +104,15 @@ void DiskGalaxySimulator::sendToDevice() {
 }
 
 void DiskGalaxySimulator::recvFromDevice() {
-  check(nb_sim_read_pos(handle, pos.x.data(), pos.y.data(), pos.z.data()),
-        "nb_sim_read_pos");
-  check(nb_sim_read_vel(handle, vel.x.data(), vel.y.data(), vel.z.data()),
-        "nb_sim_read_vel");
+  // one asynchronous pinned copy of all six arrays, one wait
+  check(nb_sim_stage_readback(handle), "nb_sim_stage_readback");
+  const float *a[6];
+  check(nb_sim_staged(handle, a), "nb_sim_staged");
+  const size_t n = params.numParticles;
+  if (n == 0) return;
+  std::vector<coords_t> *dst[6] = {&pos.x, &pos.y, &pos.z,
+                                   &vel.x, &vel.y, &vel.z};
+  for (int c = 0; c < 6; ++c) std::copy(a[c], a[c] + n, dst[c]->begin());
 }
 
 void DiskGalaxySimulator::stepSim() {

@@ -35,7 +35,7 @@ __all__ = [
     "disk_gen_ic", "forward_config_count", "forward_config_info", "forward_step", "load",
     "plummer_ic", "adjoint_config_count", "adjoint_config_info", "adjoint_step",
     "fwd_adjoint_step", "trajectory", "NbVec3", "Batch", "batch_config_count",
-    "batch_config_info", "plummer_batch_sizes", "energy_momentum", "forces_f64",
+    "batch_config_info", "plummer_batch_sizes", "energy_momentum", "forces_f64", "pack_vec4",
 ]
 
 
@@ -147,6 +147,27 @@ class Simulator:
     def positions(self):
         return self._read(self._lib.nb_sim_read_pos)
 
+    def staged(self):
+        """Positions and velocities through the pinned staging buffer:
+        ((x, y, z), (vx, vy, vz)) copies of the six arrays."""
+        check(self._lib.nb_sim_stage_readback(self._h))
+        ptrs = (ctypes.c_void_p * 6)()
+        check(self._lib.nb_sim_staged(self._h, ptrs))
+        n = self.params.num_particles
+        out = []
+        for p in ptrs:
+            a = np.empty(n, np.float32)
+            if n:
+                ctypes.memmove(a.ctypes.data, p, 4 * n)
+            out.append(a)
+        return tuple(out[:3]), tuple(out[3:])
+
+    def read_vec4(self, which: int = 0, w: float = 1.0) -> np.ndarray:
+        """(n, 4) float32: positions (which=0) or velocities (1) as vec4."""
+        out = np.empty((self.params.num_particles, 4), np.float32)
+        check(self._lib.nb_sim_read_vec4(self._h, which, w, ctypes.c_void_p(out.ctypes.data)))
+        return out
+
     def velocities(self):
         return self._read(self._lib.nb_sim_read_vel)
 
@@ -173,6 +194,20 @@ def _dev_ptrs(ts, n: int, name: str) -> list[ctypes.c_void_p]:
         if t.numel() < n:
             raise ValueError(f"{name}: tensor holds {t.numel()} < {n} values")
         out.append(ctypes.c_void_p(t.data_ptr()))
+    return out
+
+
+def pack_vec4(soa, w: float = 1.0, stream=None):
+    """(3, n) float32 CUDA tensor -> (n, 4) float32 CUDA tensor of
+    vec4(x, y, z, w), packed by the HIP kernel."""
+    import torch
+
+    from .adjoint import _stream, _vec3
+
+    n = int(soa[0].numel())
+    out = torch.empty(n, 4, dtype=torch.float32, device=soa.device)
+    check(load().nb_pack_vec4_dev(ctypes.byref(_vec3(soa)), w, ctypes.c_void_p(out.data_ptr()),
+                                  n, _stream(stream)))
     return out
 
 

@@ -113,6 +113,9 @@ def load() -> ctypes.CDLL:
     _sig(lib, "nb_sim_set_frame_iterations", c_int, [vp, c_int])
     _sig(lib, "nb_sim_set_config", c_int, [vp, c_int])
     _sig(lib, "nb_sim_config", c_int, [vp])
+    _sig(lib, "nb_sim_stage_readback", c_int, [vp])
+    _sig(lib, "nb_sim_staged", c_int, [vp, POINTER(c_void_p)])
+    _sig(lib, "nb_sim_read_vec4", c_int, [vp, c_int, c_float, vp])
     _sig(lib, "nb_sim_destroy", c_int, [vp])
     _sig(lib, "nb_forward_step_dev", c_int,
          [POINTER(NbParams)] + [vp] * 9 + [c_size_t, c_int, c_int, vp])
@@ -143,6 +146,7 @@ def load() -> ctypes.CDLL:
     _sig(lib, "nb_batch_read_pos", c_int, [vp, vp, vp, vp])
     _sig(lib, "nb_batch_read_vel", c_int, [vp, vp, vp, vp])
     _sig(lib, "nb_batch_destroy", c_int, [vp])
+    _sig(lib, "nb_pack_vec4_dev", c_int, [V3, c_float, vp, c_size_t, vp])
     _sig(lib, "nb_forces_f64_dev", c_int,
          [POINTER(NbParams), V3, vp, c_size_t, vp, vp, vp, c_size_t, vp])
     _sig(lib, "nb_energy_momentum_dev", c_int,

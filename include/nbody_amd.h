@@ -127,6 +127,22 @@ int nb_sim_sync(nb_sim *s);
 int nb_sim_read_pos(nb_sim *s, float *x, float *y, float *z);
 int nb_sim_read_vel(nb_sim *s, float *x, float *y, float *z);
 
+/* Readback through pinned host memory (SURVEY's per-frame readback path):
+ * nb_sim_stage_readback enqueues asynchronous copies of the current
+ * positions and velocities into a pinned buffer owned by the handle and
+ * returns at once; nb_sim_staged waits for them and returns six pointers
+ * (x, y, z, vx, vy, vz) valid until the next stage call or nb_sim_destroy.
+ * The reference issues six synchronous pageable copies per frame instead
+ * (src/simulator.cu:106-129). */
+int nb_sim_stage_readback(nb_sim *s);
+int nb_sim_staged(nb_sim *s, const float **arrays);
+
+/* Positions (which = 0) or velocities (which = 1) as interleaved
+ * vec4(x, y, z, w) into a host array of 4 n floats: the renderer's buffer
+ * format (src/renderer_gl.cpp:156-172, shaders/gl/main.vert:7-8), packed on
+ * the device. */
+int nb_sim_read_vec4(nb_sim *s, int which, float w, float *out);
+
 /* getLastStepTime (src/simulator.cuh:134): ms of the last nb_sim_step, from
  * before the first launch until the device synchronise. */
 float nb_sim_last_step_ms(const nb_sim *s);
@@ -253,6 +269,12 @@ size_t nb_batch_num_particles(const nb_batch *b);
 int nb_batch_read_pos(nb_batch *b, float *x, float *y, float *z);
 int nb_batch_read_vel(nb_batch *b, float *x, float *y, float *z);
 int nb_batch_destroy(nb_batch *b);
+
+/* ---- render hand-off ----------------------------------------------------- */
+/* SoA -> AoS pack: out[4i..4i+3] = (x_i, y_i, z_i, w) for i < n.  out is a
+ * device array of 4 n floats, 16 B aligned; soa and out must not overlap. */
+int nb_pack_vec4_dev(const nb_vec3 *soa, float w, float *out, size_t n,
+                     void *stream);
 
 /* ---- fp64 diagnostics (extension) --------------------------------------- */
 /* fp64 oracle force (BRANCH semantics, the step's eps) on the targets

@@ -4,6 +4,16 @@ test_oracle.py) and against central finite differences of an fp64 loss.
 
 Gradient error metric: ||g_gpu - g_fp64|| / ||g_fp64|| (2-norm over all
 particles and components); target <= 1e-3.
+
+Two kinds of reference:
+  * the fp64 adjoint evaluated along the GPU's own fp32 states (the same
+    trajectory, the same residual): this measures the adjoint's arithmetic,
+    and is the comparison for the target-matching loss of the differentiable
+    trajectory (a residual x_T - target of ~1e-3 on positions of ~50 is
+    changed by fp32 rounding of the trajectory itself, so an independent
+    fp64 trajectory gives a different, equally valid gradient);
+  * an independent fp64 trajectory and central differences, for a loss whose
+    residual is far above fp32 rounding (target 10 units away).
 """
 import numpy as np
 import pytest
@@ -115,11 +125,15 @@ def test_every_adjoint_shape(nb, gpu):
         assert rel(got, ref) < 1e-5, (cfg, nb.adjoint_config_info(cfg))
 
 
-def disk_problem(nb, orc, torch, gpu, n, steps, every):
+def disk_problem(nb, orc, torch, gpu, n, steps, every, eps=None, far=False):
     (pos, vel) = nb.disk_ic(n)
     x0 = to_dev(torch, pos, gpu)
     v0 = to_dev(torch, vel, gpu)
     params = nb.default_params(num_particles=n)
+    if eps is not None:
+        params.dist_eps = eps
+    if far:  # residual ~10: far above fp32 rounding of the positions
+        return x0, v0, params, (x0 + 10.0).contiguous()
     rng = np.random.default_rng(7)
     pert = torch.from_numpy(rng.normal(scale=1e-3, size=(3, n)).astype(np.float32)).to(gpu)
     with torch.no_grad():
@@ -127,24 +141,70 @@ def disk_problem(nb, orc, torch, gpu, n, steps, every):
     return x0, v0, params, target
 
 
-def test_trajectory_gradient_vs_fp64(nb, orc, gpu):
+def gpu_gradient(nb, torch, x0, v0, params, target, steps, every):
+    x = x0.clone().requires_grad_(True)
+    v = v0.clone().requires_grad_(True)
+    xT, _ = nb.trajectory(x, v, params, steps, every)
+    loss = ((xT.double() - target.double()) ** 2).sum(0).mean()
+    loss.backward()
+    return loss.item(), x.grad.double().cpu().numpy(), v.grad.double().cpu().numpy(), xT
+
+
+def gpu_states(nb, torch, params, x0, v0, steps):
+    """Positions x_0 .. x_T from the forward kernel, one launch per step (the
+    launch shape the trajectory's forward pass uses)."""
+    P, V = x0.clone(), v0.clone()
+    states = [P.cpu().numpy()]
+    for _ in range(steps):
+        N = torch.empty_like(P)
+        nb.forward_step(params, list(P), list(V), list(N))
+        P = N
+        states.append(P.cpu().numpy())
+    return states
+
+
+def fp64_adjoint_along(orc, p, states, target):
+    """fp64 reverse sweep through the given (fp32) states."""
+    n = states[0].shape[1]
+    lx = 2.0 * (states[-1].astype(np.float64) - target) / n
+    lv = np.zeros_like(lx)
+    for t in range(len(states) - 2, -1, -1):
+        lx, lv = orc.adjoint_step_f64(p, states[t], lx, lv)
+        lx, lv = np.stack(lx), np.stack(lv)
+    return lx, lv
+
+
+def test_trajectory_gradient_vs_fp64_same_states(nb, orc, gpu):
+    """The differentiable-trajectory loss at SimParam defaults: the GPU's
+    checkpointed adjoint against the fp64 adjoint along the same states."""
     import torch
 
     n, T, K = 2048, 40, 10
     x0, v0, params, target = disk_problem(nb, orc, torch, gpu, n, T, K)
-    x = x0.clone().requires_grad_(True)
-    v = v0.clone().requires_grad_(True)
-    xT, vT = nb.trajectory(x, v, params, T, K)
-    loss = ((xT.double() - target.double()) ** 2).sum(0).mean()
-    loss.backward()
-    gx = x.grad.double().cpu().numpy()
-    gv = v.grad.double().cpu().numpy()
-    L64, gx64, gv64 = orc.traj_grad_f64(orc.params(), x0.cpu().numpy(), v0.cpu().numpy(),
-                                        target.double().cpu().numpy(), T)
+    L, gx, gv, xT = gpu_gradient(nb, torch, x0, v0, params, target, T, K)
+    states = gpu_states(nb, torch, params, x0, v0, T)
+    assert np.array_equal(states[-1], xT.detach().cpu().numpy())  # same trajectory
+    gx64, gv64 = fp64_adjoint_along(orc, orc.params(), states, target.double().cpu().numpy())
     ex, ev = rel(gx, gx64), rel(gv, gv64)
-    print(f"N={n} T={T} K={K}: loss {loss.item():.6e} (fp64 {L64:.6e}); "
+    print(f"N={n} T={T} K={K}: loss {L:.6e}; grad vs fp64 adjoint on the same states: "
+          f"x {ex:.2e} v {ev:.2e}")
+    assert ex <= 1e-3 and ev <= 1e-3
+
+
+def test_trajectory_gradient_vs_fp64_trajectory(nb, orc, gpu):
+    """Independent fp64 trajectory + adjoint (every state kept), for a loss
+    whose residual is far above fp32 rounding."""
+    import torch
+
+    n, T, K = 2048, 40, 10
+    x0, v0, params, target = disk_problem(nb, orc, torch, gpu, n, T, K, eps=1e-2, far=True)
+    L, gx, gv, _ = gpu_gradient(nb, torch, x0, v0, params, target, T, K)
+    L64, gx64, gv64 = orc.traj_grad_f64(orc.params(dist_eps=1e-2), x0.cpu().numpy(),
+                                        v0.cpu().numpy(), target.double().cpu().numpy(), T)
+    ex, ev = rel(gx, gx64), rel(gv, gv64)
+    print(f"N={n} T={T} K={K} eps=1e-2, far target: loss {L:.6e} (fp64 {L64:.6e}); "
           f"grad rel err x {ex:.2e} v {ev:.2e}")
-    assert abs(loss.item() - L64) <= 1e-3 * L64
+    assert abs(L - L64) <= 1e-4 * L64
     assert ex <= 1e-3 and ev <= 1e-3
 
 
@@ -166,11 +226,11 @@ def test_checkpoint_interval_does_not_change_gradient(nb, orc, gpu):
 
 def test_trajectory_central_differences(nb, orc, gpu):
     """Adjoint gradient vs central differences (h = 1e-3) of the fp64 loss
-    on 32 random coordinates of x0 and v0."""
+    on 32 random coordinates of x0 and v0 (far target, eps = 1e-2)."""
     import torch
 
     n, T, K = 1024, 30, 10
-    x0, v0, params, target = disk_problem(nb, orc, torch, gpu, n, T, K)
+    x0, v0, params, target = disk_problem(nb, orc, torch, gpu, n, T, K, eps=1e-2, far=True)
     x = x0.clone().requires_grad_(True)
     v = v0.clone().requires_grad_(True)
     xT, _ = nb.trajectory(x, v, params, T, K)
@@ -180,7 +240,7 @@ def test_trajectory_central_differences(nb, orc, gpu):
     X0 = x0.double().cpu().numpy()
     V0 = v0.double().cpu().numpy()
     tgt = target.double().cpu().numpy()
-    p = orc.params()
+    p = orc.params(dist_eps=1e-2)
     rng = np.random.default_rng(11)
     h = 1e-3
     fd, an = [], []
@@ -201,4 +261,4 @@ def test_trajectory_central_differences(nb, orc, gpu):
     fd, an = np.array(fd), np.array(an)
     err = rel(an, fd)
     print(f"central differences on 32 coordinates: rel err {err:.2e}")
-    assert err <= 1e-2
+    assert err <= 1e-3

@@ -258,3 +258,34 @@ def test_reference_driver_runs(nb, gpu):
         assert res.returncode == 0, res.stderr
         lines = [l for l in res.stdout.splitlines() if l.startswith("At step")]
         assert len(lines) == 3 and lines[-1].startswith("At step 5 kernel time is ")
+
+
+def test_staged_readback_matches_plain_reads(nb, gpu):
+    n = 3001
+    pos, vel = nb.disk_ic(n)
+    sim = nb.Simulator(nb.default_params(num_particles=n, sim_iterations_per_frame=3), pos, vel)
+    sim.step_sim()
+    (sp, sv) = sim.staged()
+    for a, b in zip(sp + sv, sim.positions() + sim.velocities()):
+        assert np.array_equal(a, b)
+    sim.close()
+
+
+def test_vec4_pack(nb, gpu):
+    import torch
+
+    n = 4099
+    pos, vel = nb.disk_ic(n)
+    sim = nb.Simulator(nb.default_params(num_particles=n), pos, vel)
+    p4 = sim.read_vec4(0, 1.0)
+    v4 = sim.read_vec4(1, 0.0)
+    assert np.array_equal(p4[:, :3], np.stack(pos, axis=1)) and np.all(p4[:, 3] == 1.0)
+    assert np.array_equal(v4[:, :3], np.stack(vel, axis=1)) and np.all(v4[:, 3] == 0.0)
+    sim.close()
+    soa = torch.from_numpy(np.stack(pos)).to(gpu)
+    out = nb.pack_vec4(soa, 2.5).cpu().numpy()
+    assert np.array_equal(out[:, :3], np.stack(pos, axis=1)) and np.all(out[:, 3] == 2.5)
+    with pytest.raises(nb.NbError):  # output overlapping the input
+        import ctypes
+        nb.check(nb.load().nb_pack_vec4_dev(ctypes.byref(nb.NbVec3(*[ctypes.c_void_p(
+            soa[c].data_ptr()) for c in range(3)])), 1.0, ctypes.c_void_p(soa.data_ptr()), n, None))
