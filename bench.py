@@ -53,7 +53,23 @@ def parse_args():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the fwd+adjoint, trajectory and batched sections")
     ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (the known-size "
+                         "reference for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy(mib: int) -> int:
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n * 4, "launches": 5}), flush=True)
+    return 0
 
 
 def _events(torch, fn, reps: int) -> float:
@@ -210,6 +226,8 @@ def main() -> int:
 
     import nbody_amd as nb
 
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         return 1

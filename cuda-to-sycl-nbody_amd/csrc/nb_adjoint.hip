@@ -276,7 +276,8 @@ const FwdConfig kAdjConfigs[] = {
     {256, 1, 1024, 1}, {256, 2, 1024, 1}, {256, 2, 1024, 2},
     {256, 4, 1024, 1}, {256, 4, 1024, 2}, {256, 4, 1024, 4},
     {128, 4, 512, 2},  {128, 4, 512, 4},  {256, 1, 1024, 2},
-    {256, 2, 1024, 4},
+    {256, 2, 1024, 4}, {256, 1, 1024, 4}, {256, 1, 1024, 8},
+    {256, 2, 1024, 8}, {256, 4, 1024, 8},
 };
 constexpr int kNumAdj = sizeof(kAdjConfigs) / sizeof(kAdjConfigs[0]);
 
@@ -293,9 +294,10 @@ int adjoint_config_count() { return kNumAdj; }
 const FwdConfig &adjoint_config(int cfg) { return kAdjConfigs[cfg]; }
 
 int adjoint_default_config(size_t n) {
-  if (n >= (size_t)1 << 17) return 4; // 256 x 4, split 2
-  if (n >= (size_t)1 << 14) return 2; // 256 x 2, split 2
-  return 0;
+  // Measured on MI355X: 256 x 1 split 2 at N = 262144 (fused and plain),
+  // 256 x 2 split 4 at N = 16384..25600.
+  if (n >= (size_t)1 << 17) return 8;
+  return 9;
 }
 
 size_t adjoint_scratch_bytes(int cfg, size_t n) {

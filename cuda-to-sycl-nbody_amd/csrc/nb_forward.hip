@@ -159,10 +159,10 @@ const FwdConfig &forward_config(int cfg) { return kConfigs[cfg]; }
 
 int forward_default_config(size_t n, int acc) {
   (void)acc;
-  // Enough workgroups to fill 256 CUs several times over.
-  if (n >= (size_t)1 << 17) return 0; // 256 x 1
-  if (n >= (size_t)1 << 14) return 2; // 256 x 1, split 4
-  return 3;                           // 256 x 1, split 8
+  // Measured on MI355X (disk ICs): splitting the sources 8 ways keeps 4-8
+  // waves per SIMD in flight; large N also profits from 4 targets per thread.
+  if (n >= (size_t)1 << 17) return 11; // 256 x 4, tile 1024, split 8
+  return 3;                            // 256 x 1, tile 1024, split 8
 }
 
 size_t forward_scratch_bytes(int cfg, size_t n) {

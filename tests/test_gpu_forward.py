@@ -47,10 +47,17 @@ def fp64_step(orc, p, pos, vel, idx):
 
 
 def assert_step_parity(orc, p, pos, vel, gpu_pos, ref_pos, idx, label):
-    """gpu_pos/ref_pos: positions after one step at targets idx."""
+    """gpu_pos/ref_pos: positions after one step at targets idx.
+
+    Every particle where the GPU and the fp32 oracle differ by more than 1e-6
+    is refereed by the fp64 step: the GPU may be no farther from it than 4x
+    the oracle's distance.  The 99th percentile of the GPU's own error
+    against the fp64 step must stay within the per-step target (where the
+    two fp32 results agree to 1e-6, the oracle's distance is used)."""
     e_go = per_particle(gpu_pos, ref_pos)
     bad = np.nonzero(e_go > 1e-6)[0]
     msg = f"{label}: vs fp32 oracle max {e_go.max():.2e} p99 {np.percentile(e_go, 99):.2e}"
+    e_gpu = e_go.copy()  # upper bound of the GPU's distance where they agree
     if bad.size:
         x64 = fp64_step(orc, p, pos, vel, idx[bad])
         e_g64 = per_particle([a[bad] for a in gpu_pos], x64)
@@ -58,8 +65,10 @@ def assert_step_parity(orc, p, pos, vel, gpu_pos, ref_pos, idx, label):
         msg += (f"; {bad.size} over 1e-6: vs fp64 gpu {e_g64.max():.2e} "
                 f"oracle {e_o64.max():.2e}")
         assert np.all(e_g64 <= 4 * e_o64 + 1e-7), msg
+        e_gpu[bad] = e_g64
+    msg += f"; gpu p99 vs fp64 <= {np.percentile(e_gpu, 99):.2e}"
     print(msg)
-    assert np.percentile(e_go, 99) <= TOL_STEP, msg
+    assert np.percentile(e_gpu, 99) <= TOL_STEP, msg
 
 
 def gpu_steps(nb, params, pos, vel, steps):
