@@ -180,14 +180,23 @@ def batched_section(nb, steps: int) -> dict:
     }
 
 
-def traffic_from_profile(n: int):
-    """HBM bytes per launch of the pair kernel from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated), if one exists for n."""
-    f = ROOT / "profiles" / f"forward_n{n}_hbm_traffic.txt"
+def traffic_from_profile(scheme: str, n: int):
+    """HBM bytes per launch of the pair kernel of `scheme` from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated against a
+    copy of known size), if one exists for n."""
+    f = ROOT / "profiles" / f"forward_{scheme}_n{n}_hbm_traffic.txt"
     if not f.exists():
         return None
     m = re.search(r"^bytes_per_launch:\s*([0-9.eE+]+)", f.read_text(), re.M)
     return float(m.group(1)) if m else None
+
+
+# FLOP the hardware executes per interaction (one fused multiply-add = 2):
+#   targets:   every pair evaluated for both particles: 3 sub + 3 fma + 2 mul
+#              + rsqrt + 3 fma = 18;
+#   symmetric: each pair evaluated once for two interactions: (3 sub + 3 fma
+#              + 2 mul + rsqrt + 6 fma) / 2 = 12.
+EXECUTED_FLOP_PER_INTERACTION = {"targets": 18, "symmetric": 12}
 
 
 def cpu_baseline(pos, vel, n: int, seconds: float) -> dict:
@@ -277,13 +286,19 @@ def main() -> int:
             "launch": cfg,
         },
         "roofline": {
-            "bound": "valu_fp32",
+            "bound": "mfma",
             "achieved": round(achieved, 2),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": traffic_from_profile(n),
+            "traffic": traffic_from_profile(cfg["scheme"], n),
+            "bound_note": "fp32 compute roof: 157.3 TFLOP/s is MI355X's fp32 peak for the "
+                          "vector ALUs and the matrix cores alike; the pair loop runs on the "
+                          "vector ALUs (not a contraction)",
+            "flop_per_interaction": FLOP_PER_INTERACTION,
             "kernel_ms_per_launch": round(per_launch_ms, 4),
+            "executed_frac": round(EXECUTED_FLOP_PER_INTERACTION[cfg["scheme"]] * inter
+                                   / (per_launch_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
         },
     }
     if not args.no_extras:

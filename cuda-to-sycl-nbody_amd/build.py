@@ -29,7 +29,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["nb_forward.hip", "nb_adjoint.hip", "nb_traj.hip", "nb_batch.hip", "nb_diag.hip", "nb_pack.hip", "nb_capi.hip"]
+HIP_SOURCES = ["nb_forward.hip", "nb_sym.hip", "nb_adjoint.hip", "nb_traj.hip", "nb_batch.hip", "nb_diag.hip", "nb_pack.hip", "nb_capi.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",

@@ -315,7 +315,7 @@ int timed(nb_batch *b, int steps, bool graph, bool separate) {
         c.G = p.G;
         float *const *src = b->P[b->cur];
         float *const *dst = b->P[b->cur ^ 1];
-        st = launch_forward(c, b->mode, forward_default_config(n, NB_ACC_PLAIN),
+        st = launch_forward(c, b->mode, forward_default_config(n, b->mode),
                             src[0] + o, src[1] + o, src[2] + o, b->V[0] + o,
                             b->V[1] + o, b->V[2] + o, dst[0] + o, dst[1] + o,
                             dst[2] + o, n, b->scratch, b->stream);
@@ -421,7 +421,7 @@ int nb_batch_create(int num_systems, const nb_params *params, const float *px,
     const size_t n = params[s].num_particles;
     scratch_bytes = std::max(
         scratch_bytes,
-        forward_scratch_bytes(forward_default_config(n, NB_ACC_PLAIN), n));
+        forward_scratch_bytes(forward_default_config(n, b->mode), n));
   }
   const size_t floats = 15 * std::max<size_t>(b->stride, 64);
   if ((st = hip_check(hipMalloc(&b->mem, sizeof(float) * floats),

@@ -100,8 +100,10 @@ struct nb_sim {
   float *mem = nullptr; // 9 SoA arrays: pos, pos_next, vel
   float *pos[3] = {}, *nxt[3] = {}, *vel[3] = {};
   float *scratch = nullptr;
+  size_t scratch_bytes = 0;
   int acc = NB_ACC_PLAIN;
-  int cfg = 0;
+  int cfg = 0;      // launch shape in use
+  int cfg_req = -1; // shape asked for with nb_sim_set_config (-1: default)
   float last_ms = 0.0f, last_kernel_ms = 0.0f;
   float *staging = nullptr; // pinned host copy of pos and vel (6 arrays)
   hipEvent_t ev_stage = nullptr;
@@ -147,17 +149,28 @@ void destroy_sim(nb_sim *s) {
 }
 
 int ensure_sim_scratch(nb_sim *s) {
+  const size_t bytes = s->n ? forward_scratch_bytes(s->cfg, s->n) : 0;
+  if (bytes == s->scratch_bytes) return NB_OK;
   if (s->scratch) {
     (void)hipFree(s->scratch);
     s->scratch = nullptr;
+    s->scratch_bytes = 0;
   }
-  const size_t bytes = forward_scratch_bytes(s->cfg, s->n);
   if (bytes == 0) return NB_OK;
   void *p = nullptr;
   int st = hip_check(hipMalloc(&p, bytes), "scratch hipMalloc");
   if (st) return st;
   s->scratch = static_cast<float *>(p);
+  s->scratch_bytes = bytes;
   return NB_OK;
+}
+
+// Launch shape for the current accumulation mode and parameters: the one
+// asked for with nb_sim_set_config, else the library's choice.
+int refresh_config(nb_sim *s) {
+  const int mode = pair_mode(s->p.calc_method, s->acc, s->p.dist_eps);
+  s->cfg = s->cfg_req < 0 ? forward_default_config(s->n, mode) : s->cfg_req;
+  return ensure_sim_scratch(s);
 }
 
 } // namespace
@@ -226,7 +239,8 @@ int nb_sim_create(const nb_params *p, const float *px, const float *py,
   s->p = *p;
   s->n = n;
   s->stride = (n + 63) / 64 * 64; // every array starts 256 B aligned
-  s->cfg = forward_default_config(n, s->acc);
+  s->cfg = forward_default_config(
+      n, pair_mode(s->p.calc_method, s->acc, s->p.dist_eps));
   if ((st = hip_check(hipGetDevice(&s->device), "hipGetDevice")) ||
       (st = hip_check(hipStreamCreateWithFlags(&s->stream,
                                                hipStreamNonBlocking),
@@ -409,8 +423,10 @@ int nb_sim_set_accumulation(nb_sim *s, int mode) {
   if (!s) return set_error(NB_ERR_INVALID_ARG, "sim is NULL");
   if (mode != NB_ACC_PLAIN && mode != NB_ACC_KAHAN)
     return set_error(NB_ERR_INVALID_ARG, "unknown accumulation mode");
+  int st = hip_check(hipStreamSynchronize(s->stream), "stream synchronize");
+  if (st) return st;
   s->acc = mode;
-  return NB_OK;
+  return refresh_config(s);
 }
 
 int nb_sim_set_frame_iterations(nb_sim *s, int iters) {
@@ -426,8 +442,8 @@ int nb_sim_set_config(nb_sim *s, int cfg) {
     return set_error(NB_ERR_INVALID_ARG, "forward config out of range");
   int st = hip_check(hipStreamSynchronize(s->stream), "stream synchronize");
   if (st) return st;
-  s->cfg = cfg < 0 ? forward_default_config(s->n, s->acc) : cfg;
-  return s->n ? ensure_sim_scratch(s) : NB_OK;
+  s->cfg_req = cfg;
+  return refresh_config(s);
 }
 
 int nb_sim_config(const nb_sim *s) { return s ? s->cfg : -1; }
@@ -455,15 +471,15 @@ int nb_forward_step_dev(const nb_params *p, const float *px, const float *py,
   if (px == nx || py == ny || pz == nz)
     return set_error(NB_ERR_INVALID_ARG,
                      "positions and next positions must not alias");
-  if (cfg < 0) cfg = forward_default_config(n, acc);
+  const int mode = pair_mode(p->calc_method, acc, p->dist_eps);
+  if (cfg < 0) cfg = forward_default_config(n, mode);
   float *scratch = nullptr;
   const size_t bytes = forward_scratch_bytes(cfg, n);
   if (bytes) {
     scratch = device_scratch(bytes, &st);
     if (st) return st;
   }
-  return launch_forward(consts_of(p),
-                        pair_mode(p->calc_method, acc, p->dist_eps), cfg, px,
+  return launch_forward(consts_of(p), mode, cfg, px,
                         py, pz, vx, vy, vz, nx, ny, nz, (int)n, scratch,
                         static_cast<hipStream_t>(stream));
 }
@@ -480,6 +496,14 @@ int nb_forward_config_info(int cfg, int *block, int *unroll, int *tile,
   if (tile) *tile = c.tile;
   if (split) *split = c.split;
   return NB_OK;
+}
+
+int nb_forward_config_scheme(int cfg) {
+  if (cfg < 0 || cfg >= forward_config_count()) {
+    set_error(NB_ERR_INVALID_ARG, "forward config out of range");
+    return -1;
+  }
+  return forward_config(cfg).scheme;
 }
 
 } // extern "C"

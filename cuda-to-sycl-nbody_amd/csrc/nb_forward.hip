@@ -12,7 +12,7 @@
 //     accumulate_sources), read back as wave-uniform b128 broadcasts;
 //   * optional source split (grid.y = S chunks) writes partial forces and a
 //     second kernel sums the S partials in chunk order and integrates.
-// The pair arithmetic (10 VALU + rsq + rcp per interaction) is in
+// The pair arithmetic (11 VALU + v_rsq_f32 per interaction) is in
 // nb_pairs.hpp.  No matrix cores: the pair loop is not a dense contraction.
 //
 // Build with -ffp-contract=off: the integrator must round each product and
@@ -130,8 +130,10 @@ const ShapeEntry kShapes[] = {
     {512, 2, 2048, shape_kernel<512, 2, 2048>},
 };
 
-// Compiled launch shapes: {block, unroll, tile, split}.  Index 0 must stay
-// the unsplit 256 x 1 shape (a valid choice for every n).
+// Compiled launch shapes: {block, unroll, tile, split[, scheme]}.  Index 0
+// must stay the unsplit 256 x 1 shape (a valid choice for every n).  The
+// symmetric shapes (nb_sym.hip) are {256, particles per lane, super-tile,
+// super-tiles per work item}.
 const FwdConfig kConfigs[] = {
     {256, 1, 1024, 1}, {256, 1, 1024, 2}, {256, 1, 1024, 4},
     {256, 1, 1024, 8}, {256, 2, 1024, 1}, {256, 2, 1024, 2},
@@ -141,6 +143,9 @@ const FwdConfig kConfigs[] = {
     {128, 2, 512, 2},  {128, 2, 512, 4},  {512, 1, 2048, 1},
     {512, 1, 2048, 2}, {512, 2, 2048, 1}, {512, 2, 2048, 2},
     {512, 2, 2048, 4},
+    {256, 8, 2048, 1, SCHEME_SYMMETRIC}, {256, 8, 2048, 2, SCHEME_SYMMETRIC},
+    {256, 8, 2048, 4, SCHEME_SYMMETRIC}, {256, 4, 1024, 1, SCHEME_SYMMETRIC},
+    {256, 4, 1024, 2, SCHEME_SYMMETRIC}, {256, 4, 1024, 4, SCHEME_SYMMETRIC},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
@@ -157,16 +162,26 @@ int forward_config_count() { return kNumConfigs; }
 
 const FwdConfig &forward_config(int cfg) { return kConfigs[cfg]; }
 
-int forward_default_config(size_t n, int acc) {
-  (void)acc;
-  // Measured on MI355X (disk ICs): splitting the sources 8 ways keeps 4-8
-  // waves per SIMD in flight; large N also profits from 4 targets per thread.
+int forward_default_config(size_t n, int mode) {
+  // Measured on MI355X (disk ICs, symmetric vs target-sharded shapes):
+  //   n = 2^16: symmetric 4 per lane 4214 Gint/s, targets 3402;
+  //   n = 2^17 .. 2^20: symmetric 8 per lane 5048 .. 5803, targets 3657 .. 3958.
+  // Past 2^19 two super-tiles per item run as fast and need 25 % less
+  // scratch.  Without the symmetric scheme, splitting the sources 8 ways keeps
+  // 4-8 waves per SIMD in flight, and large n also profits from 4 targets per
+  // thread.
+  if (mode == MODE_PLAIN && n <= kSymMaxParticles) {
+    if (n > ((size_t)1 << 19)) return 23; // symmetric, 8 per lane, 2 per item
+    if (n >= ((size_t)1 << 17)) return 22; // symmetric, 8 per lane, 1 per item
+    if (n >= ((size_t)1 << 15)) return 25; // symmetric, 4 per lane, 1 per item
+  }
   if (n >= (size_t)1 << 17) return 11; // 256 x 4, tile 1024, split 8
   return 3;                            // 256 x 1, tile 1024, split 8
 }
 
 size_t forward_scratch_bytes(int cfg, size_t n) {
   const FwdConfig &c = kConfigs[cfg];
+  if (c.scheme == SCHEME_SYMMETRIC) return sym_scratch_bytes(c.unroll, c.split, n);
   return c.split > 1 ? sizeof(float) * 3 * (size_t)c.split * n : 0;
 }
 
@@ -178,6 +193,14 @@ int launch_forward(const StepConsts &c, int mode, int cfg, const float *px,
     return set_error(NB_ERR_INVALID_ARG, "forward config out of range");
   if (n <= 0) return NB_OK;
   const FwdConfig &fc = kConfigs[cfg];
+  if (fc.scheme == SCHEME_SYMMETRIC) {
+    if (mode != MODE_PLAIN)
+      return set_error(NB_ERR_UNSUPPORTED,
+                       "symmetric launch shapes take plain accumulation with "
+                       "dist_eps^3 a normal float");
+    return launch_forward_sym(c, fc.unroll, fc.split, px, py, pz, vx, vy, vz,
+                              nx, ny, nz, n, scratch, stream);
+  }
   KernelFn fn = find_kernel(fc, mode);
   if (!fn) return set_error(NB_ERR_UNSUPPORTED, "no kernel for this mode");
   if (fc.split > 1 && !scratch)

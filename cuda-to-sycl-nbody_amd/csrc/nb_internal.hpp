@@ -39,18 +39,40 @@ enum PairMode : int {
   MODE_PREDICATED = 4,
 };
 
-// One compiled launch shape of the forward kernel.
+// Forward decomposition schemes.
+enum FwdScheme : int {
+  SCHEME_TARGETS = 0,   // each thread sums all sources for its targets
+  SCHEME_SYMMETRIC = 1, // each pair term evaluated once, applied to both
+};
+
+// One compiled launch shape of the forward kernel.  For SCHEME_SYMMETRIC
+// (nb_sym.hip): block = 256 (4 waves), unroll = particles per lane,
+// tile = super-tile length (4 * 64 * unroll), split = super-tiles per work
+// item (row chunk).
 struct FwdConfig {
   int block;  // threads per workgroup
   int unroll; // target particles per thread
   int tile;   // source particles staged in LDS per pass
   int split;  // source-range chunks (grid.y); >1 writes partial forces
+  int scheme = SCHEME_TARGETS;
 };
 
 int forward_config_count();
 const FwdConfig &forward_config(int cfg);
-// Library's launch shape for n particles.
-int forward_default_config(size_t n, int acc);
+// Library's launch shape for n particles in pair mode `mode` (MODE_*).
+int forward_default_config(size_t n, int mode);
+
+// Largest particle count of the symmetric scheme (its partial buffers grow
+// as n^2: 4.0 GB at 2^20 with 8 particles per lane and 4-tile chunks).
+constexpr size_t kSymMaxParticles = (size_t)1 << 20;
+
+// Symmetric scheme (MODE_PLAIN only): scratch bytes and stream-ordered
+// launch of one step.
+size_t sym_scratch_bytes(int unroll, int chunk, size_t n);
+int launch_forward_sym(const StepConsts &c, int unroll, int chunk,
+                       const float *px, const float *py, const float *pz,
+                       float *vx, float *vy, float *vz, float *nx, float *ny,
+                       float *nz, int n, float *scratch, hipStream_t stream);
 
 // Stream-ordered launch of one step on device SoA buffers.  `scratch` must
 // hold 3 * split * n floats when the shape has split > 1 (else unused).

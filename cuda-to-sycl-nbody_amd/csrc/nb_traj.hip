@@ -93,10 +93,13 @@ TrajLayout traj_layout(size_t n, int every) {
   L.lam_a = L.replay + (size_t)every * n3;
   L.lam_b = L.lam_a + 2 * n3;
   L.scratch = L.lam_b + 2 * n3;
-  const int cf = forward_default_config(n, NB_ACC_PLAIN);
+  // the forward shape depends on the pair mode: size for both candidates
+  const int cf = forward_default_config(n, MODE_PLAIN);
+  const int cs = forward_default_config(n, MODE_PLAIN_SKIP);
   const int ca = adjoint_default_config(n);
-  const size_t sb =
-      std::max(forward_scratch_bytes(cf, n), adjoint_scratch_bytes(ca, n));
+  const size_t sb = std::max({forward_scratch_bytes(cf, n),
+                              forward_scratch_bytes(cs, n),
+                              adjoint_scratch_bytes(ca, n)});
   L.total = L.scratch + (sb + sizeof(float) - 1) / sizeof(float);
   return L;
 }
@@ -110,7 +113,7 @@ int copy_d2d(float *dst, const float *src, size_t floats, hipStream_t s) {
 int forward_on(const nb_params *p, float *from, float *to, float *vel,
                size_t n, float *scratch, hipStream_t s) {
   const int mode = pair_mode(p->calc_method, NB_ACC_PLAIN, p->dist_eps);
-  const int cfg = forward_default_config(n, NB_ACC_PLAIN);
+  const int cfg = forward_default_config(n, mode);
   return launch_forward(consts(p), mode, cfg, from, from + n, from + 2 * n,
                         vel, vel + n, vel + 2 * n, to, to + n, to + 2 * n,
                         (int)n, scratch, s);

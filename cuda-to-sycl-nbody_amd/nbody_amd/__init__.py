@@ -84,10 +84,23 @@ def forward_config_count() -> int:
     return int(load().nb_forward_config_count())
 
 
+SCHEMES = {0: "targets", 1: "symmetric"}
+
+
 def forward_config_info(cfg: int) -> dict:
+    """Launch shape `cfg`: block, unroll, tile, split and scheme ("targets":
+    each thread sums all sources for its targets; "symmetric": each pair term
+    evaluated once and applied to both particles, where tile is the
+    super-tile length and split the super-tiles per work item)."""
+    lib = load()
     vals = [ctypes.c_int() for _ in range(4)]
-    check(load().nb_forward_config_info(cfg, *[ctypes.byref(v) for v in vals]))
-    return dict(zip(("block", "unroll", "tile", "split"), (v.value for v in vals)))
+    check(lib.nb_forward_config_info(cfg, *[ctypes.byref(v) for v in vals]))
+    info = dict(zip(("block", "unroll", "tile", "split"), (v.value for v in vals)))
+    scheme = int(lib.nb_forward_config_scheme(cfg))
+    if scheme < 0:
+        check(NB_ERR_INVALID_ARG)
+    info["scheme"] = SCHEMES[scheme]
+    return info
 
 
 class Simulator:

@@ -153,6 +153,7 @@ def load() -> ctypes.CDLL:
          [POINTER(NbParams), V3, V3, c_size_t, vp, vp])
     _sig(lib, "nb_forward_config_count", c_int, [])
     _sig(lib, "nb_forward_config_info", c_int, [c_int] + [POINTER(c_int)] * 4)
+    _sig(lib, "nb_forward_config_scheme", c_int, [c_int])
     _lib = lib
     return lib
 

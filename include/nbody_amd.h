@@ -179,10 +179,25 @@ int nb_forward_step_dev(const nb_params *p, const float *px, const float *py,
                         int acc, void *stream);
 
 /* Number of compiled forward launch shapes, and a description of one:
- * block threads, targets per thread, LDS tile length, j-split count. */
+ * block threads, targets per thread, LDS tile length, j-split count.
+ * For a symmetric shape: 256 threads, particles per lane, super-tile length,
+ * super-tiles per work item. */
 int nb_forward_config_count(void);
 int nb_forward_config_info(int cfg, int *block, int *unroll, int *tile,
                            int *split);
+
+/* Decomposition of a forward launch shape (same step, same per-pair
+ * arithmetic; no counterpart in the reference, whose kernel is the
+ * target-sharded one):
+ *   NB_SCHEME_TARGETS    each thread sums all sources for its targets;
+ *   NB_SCHEME_SYMMETRIC  each pair term is evaluated once and applied to both
+ *                        particles (Newton's third law); plain accumulation
+ *                        with dist_eps^3 a normal float, n <= 2^20, scratch
+ *                        growing as n^2 (0.4 GB at n = 2^18, 4.8 GB at 2^20).
+ * Returns the scheme, or -1 (and sets nb_last_error) for a bad index. */
+#define NB_SCHEME_TARGETS 0
+#define NB_SCHEME_SYMMETRIC 1
+int nb_forward_config_scheme(int cfg);
 
 
 /* ---- adjoint (extension; no reference counterpart) ---------------------- */

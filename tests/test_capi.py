@@ -165,9 +165,17 @@ def test_no_device_is_reported(nb):
 def test_forward_configs_are_described(nb):
     n = nb.forward_config_count()
     assert n >= 4
+    schemes = set()
     for c in range(n):
         info = nb.forward_config_info(c)
         assert info["tile"] % info["block"] == 0 and info["split"] >= 1
+        schemes.add(info["scheme"])
+        if info["scheme"] == "symmetric":  # 4 waves, super-tile = 4 x 64 x unroll
+            assert info["block"] == 256 and info["tile"] == 256 * info["unroll"]
+    assert nb.forward_config_info(0)["scheme"] == "targets"
+    assert schemes == {"targets", "symmetric"}
+    lib = nb.load()
+    assert lib.nb_forward_config_scheme(n) == -1 and lib.nb_forward_config_scheme(-1) == -1
 
 
 def test_driver_rejects_bad_method(nb):

@@ -226,13 +226,33 @@ def test_frame_timing(nb, gpu):
     assert "MI3" in nb.device_name() or nb.device_name() != "Unknown Device"
 
 
-def test_full_size_sampled_steps(nb, orc, gpu):
+def sym_configs(nb):
+    return [c for c in range(nb.forward_config_count())
+            if nb.forward_config_info(c)["scheme"] == "symmetric"]
+
+
+def target_config(nb, n):
+    """The library's target-sharded choice for large n (4 x 256, split 8)."""
+    for c in range(nb.forward_config_count()):
+        i = nb.forward_config_info(c)
+        if i["scheme"] == "targets" and (i["block"], i["unroll"], i["split"]) == (256, 4, 8):
+            return c
+    raise AssertionError("no 256 x 4 split-8 shape")
+
+
+@pytest.mark.parametrize("cfg", [-1, "targets"])
+def test_full_size_sampled_steps(nb, orc, gpu, cfg):
     """N = 262144 (1024 x 256): step 1 and step 10 (from the GPU's own state
-    at step 9) checked on two 2048-particle target ranges."""
+    at step 9) checked on two 2048-particle target ranges, for the library's
+    default shape (the symmetric scheme) and the target-sharded one."""
     n = 262144
     pos, vel = nb.disk_ic(n)
     params = nb.default_params(num_particles=n, sim_iterations_per_frame=1)
     sim = nb.Simulator(params, pos, vel)
+    if cfg == "targets":
+        sim.set_config(target_config(nb, n))
+    else:
+        assert nb.forward_config_info(sim.config)["scheme"] == "symmetric"
     p = orc.params(num_particles=n)
     ranges = [(0, 2048), (200000, 202048)]
     prev = (pos, vel)
@@ -298,3 +318,105 @@ def test_vec4_pack(nb, gpu):
         import ctypes
         nb.check(nb.load().nb_pack_vec4_dev(ctypes.byref(nb.NbVec3(*[ctypes.c_void_p(
             soa[c].data_ptr()) for c in range(3)])), 1.0, ctypes.c_void_p(soa.data_ptr()), n, None))
+
+
+@pytest.mark.parametrize("n", [1, 2, 255, 1024, 2047, 2048, 2049, 6145, 70001])
+def test_symmetric_scheme_sizes(nb, orc, gpu, n):
+    """Every symmetric shape at sizes around its tile and super-tile lengths
+    (padding particles must add exact zeros), checked against the oracle on
+    the first, a middle and the last 1500 targets."""
+    import torch
+
+    pos, vel = nb.plummer_ic(n, seed=n, g_mass=1.0)
+    params = nb.default_params(num_particles=n, G=1.0 / n)
+    p = orc.params(num_particles=n, G=1.0 / n)
+    ranges = sorted({(0, min(n, 1500)), (max(0, n // 2 - 750), min(n, n // 2 + 750)),
+                     (max(0, n - 1500), n)})
+    refs = {r: orc.step_f32(p, pos, vel, *r)[0] for r in ranges}
+    for cfg in sym_configs(nb):
+        dp, dv = dev3(torch, pos, gpu), dev3(torch, vel, gpu)
+        dn = [torch.empty_like(t) for t in dp]
+        nb.forward_step(params, dp, dv, dn, cfg=cfg)
+        torch.cuda.synchronize()
+        got = [t.cpu().numpy() for t in dn]
+        assert np.isfinite(np.stack(got)).all()
+        for (i0, i1), rp in refs.items():
+            sl = slice(i0, i1)
+            assert_step_parity(orc, p, pos, vel, [a[sl] for a in got], [a[sl] for a in rp],
+                               np.arange(i0, i1),
+                               f"N={n} cfg {cfg} {nb.forward_config_info(cfg)} [{i0},{i1})")
+
+
+def test_symmetric_scheme_is_deterministic(nb, gpu):
+    """Partials are summed in a fixed order: repeated steps are bit-identical."""
+    import torch
+
+    n = 100000
+    pos, vel = nb.disk_ic(n)
+    params = nb.default_params(num_particles=n)
+    outs = []
+    for cfg in sym_configs(nb)[:2] * 2:
+        dp, dv = dev3(torch, pos, gpu), dev3(torch, vel, gpu)
+        dn = [torch.empty_like(t) for t in dp]
+        nb.forward_step(params, dp, dv, dn, cfg=cfg)
+        torch.cuda.synchronize()
+        outs.append(np.stack([t.cpu().numpy() for t in dn + dv]))
+    h = len(outs) // 2
+    for a, b in zip(outs[:h], outs[h:]):
+        assert np.array_equal(a, b)
+
+
+def test_symmetric_scheme_momentum(nb, orc, gpu):
+    """Each pair term enters both sums with opposite sign, so the total force
+    of the symmetric scheme is a sum of rounding residues: it must be far
+    below the total force magnitude and no larger than the fp32 oracle's."""
+    import torch
+
+    n = 20000
+    pos, vel = nb.plummer_ic(n, seed=11, g_mass=1.0)
+    params = nb.default_params(num_particles=n, G=1.0, dt=1.0, damping=0.0)
+    cfg = sym_configs(nb)[0]
+    dp = dev3(torch, pos, gpu)
+    dv = [torch.zeros(n, device=gpu) for _ in range(3)]
+    dn = [torch.empty_like(t) for t in dp]
+    nb.forward_step(params, dp, dv, dn, cfg=cfg)  # v' = F with these params
+    torch.cuda.synchronize()
+    F = np.stack([t.cpu().numpy() for t in dv]).astype(np.float64)
+    p = orc.params(num_particles=n, G=1.0, dt=1.0, damping=0.0)
+    _, rv = orc.step_f32(p, pos, [np.zeros(n, np.float32)] * 3)
+    Fo = np.stack(rv).astype(np.float64)
+    scale = np.abs(F).sum(1)
+    net, net_o = np.abs(F.sum(1)), np.abs(Fo.sum(1))
+    print(f"net force / sum |F|: symmetric {net / scale}, oracle {net_o / scale}")
+    assert np.all(net <= 1e-5 * scale)
+    assert np.all(net <= 2 * net_o + 1e-7 * scale)
+
+
+def test_symmetric_scheme_rejects_other_modes(nb, gpu):
+    import torch
+
+    cfg = sym_configs(nb)[0]
+    n = 3000
+    pos, vel = nb.disk_ic(n)
+    for kw, acc in (({}, nb.NB_ACC_KAHAN), ({"dist_eps": 0.0}, nb.NB_ACC_PLAIN),
+                    ({"calc_method": nb.NB_CALC_PREDICATED}, nb.NB_ACC_PLAIN)):
+        params = nb.default_params(num_particles=n, **kw)
+        dp, dv = dev3(torch, pos, gpu), dev3(torch, vel, gpu)
+        dn = [torch.empty_like(t) for t in dp]
+        with pytest.raises(nb.NbError, match="symmetric"):
+            nb.forward_step(params, dp, dv, dn, cfg=cfg, acc=acc)
+    # the simulator's default shape follows the accumulation mode
+    sim = nb.Simulator(nb.default_params(num_particles=70000), *nb.disk_ic(70000))
+    assert nb.forward_config_info(sim.config)["scheme"] == "symmetric"
+    sim.set_accumulation(nb.NB_ACC_KAHAN)
+    assert nb.forward_config_info(sim.config)["scheme"] == "targets"
+    sim.step_sim()
+    sim.set_accumulation(nb.NB_ACC_PLAIN)
+    assert nb.forward_config_info(sim.config)["scheme"] == "symmetric"
+    sim.close()
+    # beyond 2^20 particles: refused before any launch
+    m = (1 << 20) + 1
+    big = [torch.zeros(m, device=gpu) for _ in range(9)]
+    with pytest.raises(nb.NbError, match="2\\^20"):
+        nb.forward_step(nb.default_params(num_particles=m), big[:3], big[3:6], big[6:],
+                        cfg=cfg)
