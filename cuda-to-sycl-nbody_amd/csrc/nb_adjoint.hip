@@ -272,12 +272,16 @@ const AdjShape kAdjShapes[] = {
     {128, 4, 512, adj_shape<128, 4, 512>},
 };
 
+// {block, unroll, tile, split[, scheme]}; the symmetric shapes (nb_sym.hip)
+// are {256, particles per lane, super-tile, super-tiles per work item}.
 const FwdConfig kAdjConfigs[] = {
     {256, 1, 1024, 1}, {256, 2, 1024, 1}, {256, 2, 1024, 2},
     {256, 4, 1024, 1}, {256, 4, 1024, 2}, {256, 4, 1024, 4},
     {128, 4, 512, 2},  {128, 4, 512, 4},  {256, 1, 1024, 2},
     {256, 2, 1024, 4}, {256, 1, 1024, 4}, {256, 1, 1024, 8},
     {256, 2, 1024, 8}, {256, 4, 1024, 8},
+    {256, 4, 1024, 1, SCHEME_SYMMETRIC}, {256, 4, 1024, 2, SCHEME_SYMMETRIC},
+    {256, 8, 2048, 1, SCHEME_SYMMETRIC},
 };
 constexpr int kNumAdj = sizeof(kAdjConfigs) / sizeof(kAdjConfigs[0]);
 
@@ -293,15 +297,22 @@ AdjFn find_adj(const FwdConfig &c, bool skip, bool fused) {
 int adjoint_config_count() { return kNumAdj; }
 const FwdConfig &adjoint_config(int cfg) { return kAdjConfigs[cfg]; }
 
-int adjoint_default_config(size_t n) {
-  // Measured on MI355X: 256 x 1 split 2 at N = 262144 (fused and plain),
+int adjoint_default_config(size_t n, bool skip) {
+  // Measured on MI355X (disk ICs), adjoint / fused ms, symmetric 4 per lane
+  // vs the best target-sharded shape: n = 16384: 0.243 / 0.298 vs
+  // 0.428 / 0.471;  65536: 1.89 / 2.30 vs 2.69 / 3.09;  262144: 26.4 / 32.9
+  // vs 38.7 / 43.5.  Target-sharded: 256 x 1 split 2 at N = 262144,
   // 256 x 2 split 4 at N = 16384..25600.
+  if (!skip && n >= ((size_t)1 << 14) && n <= kSymMaxParticles)
+    return 14; // symmetric, 4 per lane, 1 super-tile per item
   if (n >= (size_t)1 << 17) return 8;
   return 9;
 }
 
 size_t adjoint_scratch_bytes(int cfg, size_t n) {
   const FwdConfig &c = kAdjConfigs[cfg];
+  if (c.scheme == SCHEME_SYMMETRIC) // sized for the fused variant
+    return sym_scratch_bytes(c.unroll, c.split, n, 6);
   return c.split > 1 ? sizeof(float) * 6 * (size_t)c.split * n : 0;
 }
 
@@ -315,6 +326,14 @@ int launch_adjoint(const StepConsts &c, bool skip, bool fused, int cfg,
     return set_error(NB_ERR_INVALID_ARG, "adjoint config out of range");
   if (n <= 0) return NB_OK;
   const FwdConfig &fc = kAdjConfigs[cfg];
+  if (fc.scheme == SCHEME_SYMMETRIC) {
+    if (skip)
+      return set_error(NB_ERR_UNSUPPORTED,
+                       "symmetric launch shapes need dist_eps^3 a normal "
+                       "float");
+    return launch_adjoint_sym(c, fused, fc.unroll, fc.split, pos, lxn, lvn,
+                              lx, lv, vel, nxt, n, scratch, stream);
+  }
   AdjFn fn = find_adj(fc, skip, fused);
   if (!fn) return set_error(NB_ERR_UNSUPPORTED, "no adjoint kernel for shape");
   if (fc.split > 1 && !scratch)

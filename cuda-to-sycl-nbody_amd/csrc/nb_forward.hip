@@ -163,9 +163,10 @@ int forward_config_count() { return kNumConfigs; }
 const FwdConfig &forward_config(int cfg) { return kConfigs[cfg]; }
 
 int forward_default_config(size_t n, int mode) {
-  // Measured on MI355X (disk ICs, symmetric vs target-sharded shapes):
-  //   n = 2^16: symmetric 4 per lane 4214 Gint/s, targets 3402;
-  //   n = 2^17 .. 2^20: symmetric 8 per lane 5048 .. 5803, targets 3657 .. 3958.
+  // Measured on MI355X (disk ICs), Gint/s, symmetric vs target-sharded:
+  //   n = 16384: 4 per lane 1767 vs split-8 2477;   32768: 2844 vs 2931;
+  //   n = 49152: 4 per lane 4023 vs 3227;           65536: 4214 vs 3402;
+  //   n = 2^17 .. 2^20: 8 per lane 5048 .. 5803 vs 3657 .. 3958.
   // Past 2^19 two super-tiles per item run as fast and need 25 % less
   // scratch.  Without the symmetric scheme, splitting the sources 8 ways keeps
   // 4-8 waves per SIMD in flight, and large n also profits from 4 targets per
@@ -173,7 +174,7 @@ int forward_default_config(size_t n, int mode) {
   if (mode == MODE_PLAIN && n <= kSymMaxParticles) {
     if (n > ((size_t)1 << 19)) return 23; // symmetric, 8 per lane, 2 per item
     if (n >= ((size_t)1 << 17)) return 22; // symmetric, 8 per lane, 1 per item
-    if (n >= ((size_t)1 << 15)) return 25; // symmetric, 4 per lane, 1 per item
+    if (n >= 40960) return 25;             // symmetric, 4 per lane, 1 per item
   }
   if (n >= (size_t)1 << 17) return 11; // 256 x 4, tile 1024, split 8
   return 3;                            // 256 x 1, tile 1024, split 8
@@ -181,7 +182,8 @@ int forward_default_config(size_t n, int mode) {
 
 size_t forward_scratch_bytes(int cfg, size_t n) {
   const FwdConfig &c = kConfigs[cfg];
-  if (c.scheme == SCHEME_SYMMETRIC) return sym_scratch_bytes(c.unroll, c.split, n);
+  if (c.scheme == SCHEME_SYMMETRIC)
+    return sym_scratch_bytes(c.unroll, c.split, n, 3);
   return c.split > 1 ? sizeof(float) * 3 * (size_t)c.split * n : 0;
 }
 

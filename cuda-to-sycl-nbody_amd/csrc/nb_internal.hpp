@@ -66,9 +66,16 @@ int forward_default_config(size_t n, int mode);
 // as n^2: 4.0 GB at 2^20 with 8 particles per lane and 4-tile chunks).
 constexpr size_t kSymMaxParticles = (size_t)1 << 20;
 
-// Symmetric scheme (MODE_PLAIN only): scratch bytes and stream-ordered
-// launch of one step.
-size_t sym_scratch_bytes(int unroll, int chunk, size_t n);
+// Symmetric scheme (nb_sym.hip; vanishing self term only): scratch bytes
+// for `acc` sums per particle (3 forward and adjoint, 6 fused adjoint) and
+// stream-ordered launches of one forward step / one adjoint step.
+size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc);
+int launch_adjoint_sym(const StepConsts &c, bool fused, int unroll, int chunk,
+                       const float *const pos[3], const float *const lxn[3],
+                       const float *const lvn[3], float *const lx[3],
+                       float *const lv[3], float *const vel[3],
+                       float *const nxt[3], int n, float *scratch,
+                       hipStream_t stream);
 int launch_forward_sym(const StepConsts &c, int unroll, int chunk,
                        const float *px, const float *py, const float *pz,
                        float *vx, float *vy, float *vz, float *nx, float *ny,
@@ -99,10 +106,12 @@ int pair_mode(int calc_method, int acc, float eps);
 // needed): dist_eps > 0 with a normal or infinite eps^3.
 bool self_term_vanishes(float eps);
 
-// Adjoint launch shapes (same fields as the forward ones).
+// Adjoint launch shapes (same fields as the forward ones).  skip: the self
+// term needs masking (dist_eps^3 not a normal float), which rules out the
+// symmetric shapes.
 int adjoint_config_count();
 const FwdConfig &adjoint_config(int cfg);
-int adjoint_default_config(size_t n);
+int adjoint_default_config(size_t n, bool skip);
 size_t adjoint_scratch_bytes(int cfg, size_t n);
 
 // Adjoint of one step at positions pos: reads the cotangents of the step's

@@ -1,43 +1,52 @@
-// Symmetric all-pairs forward step for gfx950 (MI355X): each pair term is
-// evaluated once and applied to both particles (Newton's third law).
+// Symmetric all-pairs kernels for gfx950 (MI355X): each pair term is
+// evaluated once and applied to both particles of the pair.
 //
-// Same step as particle_interaction<BRANCH> (src/simulator.cu:186-229) in the
-// parameter range where the self term vanishes (dist_eps^3 a normal float):
-//   F_i = sum_j r_ij * rsqrt(d^3),  r_ij = x_j - x_i,  d = |r_ij|^2 + eps,
-// with the reference's per-pair arithmetic (nb_pairs.hpp: pair<MODE_PLAIN>).
-// The pair term t_ij = r_ij * K_ij is added to F_i with fma(r, K, F) and to
-// F_j with fma(-r, K, S): both are the correctly rounded sum with the exact
-// product, so t_ji = -t_ij holds bit for bit.  Only the order in which the
-// terms of F_i are summed differs from the target-sharded kernel.
+// Two pair rules share one decomposition:
+//   ForceRule    the forward step, particle_interaction<BRANCH>
+//                (src/simulator.cu:186-229), in the parameter range where the
+//                self term vanishes (dist_eps^3 a normal float):
+//                  F_i = sum_j r_ij K_ij,  r_ij = x_j - x_i,
+//                  K = rsqrt(d^3),  d = |r|^2 + eps  (nb_pairs.hpp arithmetic);
+//   AdjointRule  the reverse-mode step of the same integrator (nb_adjoint.hip
+//                derives it): A_k = sum_j K (h + t3 r),  h = g_j - g_k,
+//                t3 = -3 (r . h) / d, optionally fused with the forward F.
+// Both pair terms are odd under exchanging the two particles (r -> -r,
+// h -> -h, d, K, t3 unchanged), and fma(-u, K, S) is the exact negation of
+// fma(u, K, A), so the term added to particle j is bit for bit the negative
+// of the one added to particle i.  Only the order in which each particle's
+// terms are summed differs from the target-sharded kernels.
 //
 // Decomposition (U particles per lane, P = 64 U per wave, ST = 4 P):
 //   * particles are grouped in super-tiles of ST; a work item is one
 //     workgroup (4 waves) and a chunk of one row of the super-tile pair
 //     triangle: super-tile GA against super-tiles GB in [GB0, GB1), GB >= GA;
-//   * wave w holds targets 4 GA + w (tile of P) in VGPRs, the chunk's source
-//     super-tile sits in LDS;
+//   * wave w holds the inputs and sums of targets 4 GA + w (tile of P) in
+//     VGPRs; the source super-tile sits in LDS;
 //   * for GB > GA, in pass b wave w meets source tile j = (w + b) & 3; in
 //     each of 64 steps lane l takes the U sources of set (l + s) & 63 from
-//     LDS and evaluates the U x U pair terms, adding the reaction -t to U
+//     LDS and evaluates the U x U pair terms, adding the reaction to U
 //     per-source sums that rotate one lane down the wave (DPP wave_rol:1)
 //     after each step, so that after 64 steps lane l again holds set l's
 //     sums.  Between passes the reaction sums of the four tiles live in LDS
-//     (each tile is owned by one wave per pass), and after pass 3 the chunk
+//     (each tile is owned by one wave per pass); after pass 3 the chunk
 //     writes them as the (GA, GB) reaction partial;
-//   * for GB == GA (first super-tile of a row) every wave sums the forces of
+//   * for GB == GA (first super-tile of a row) every wave sums the terms of
 //     all four tiles on its targets without reactions (self terms are +0);
-//   * the targets' forces over the chunk are one force partial;
-//   * a finalize kernel sums, per particle, its row's force partials in
-//     chunk order and then the reaction partials of its column in GA order,
-//     and integrates (integrate_ref).  The order is fixed, so results are
-//     deterministic.
-// Padding particles (index >= n) sit at 1e30 in every coordinate: any pair
-// with a real particle has |r|^2 = +inf, rsqrt(inf) = 0 and r * 0 = 0, so
-// they add exact zeros without a mask in the pair loop.
+//   * the targets' sums over the chunk are one row partial;
+//   * a finalize kernel sums, per particle, its row partials in chunk order
+//     and then the reaction partials of its column in GA order, and applies
+//     the rule's epilogue.  The order is fixed: results are deterministic.
+// Padding (index >= n): ForceRule puts padding particles at 1e30 in every
+// coordinate, so any pair with a real particle has |r|^2 = +inf,
+// rsqrt(inf) = 0 and r * 0 = 0 (exact zeros, no mask).  AdjointRule masks
+// K and t3 of pairs with a padding particle in the items that contain the
+// last, partial super-tile (a product r . h of a far padding particle could
+// overflow).
 //
-// Per pair term: 14 VALU (3 sub, 3 fma, 2 mul, 6 fma) + v_rsq_f32 + 3/U DPP
-// moves, against 11 VALU per interaction for the target-sharded kernel that
-// evaluates each pair twice.
+// Per pair term (two interactions): forward 14 VALU + v_rsq_f32 + 3/U DPP
+// moves; adjoint 25 VALU + v_rsq_f32 + v_rcp_f32 + 3/U DPP (fused: 31 VALU,
+// 6/U DPP).  The target-sharded kernels spend 11 (forward) and 22 (adjoint)
+// VALU per interaction.
 //
 // Build with -ffp-contract=off (explicit fmaf only).
 
@@ -47,7 +56,7 @@ namespace nb {
 namespace {
 
 constexpr int kWaves = 4;        // waves per workgroup = tiles per super-tile
-constexpr float kFar = 1.0e30f;  // coordinate of padding particles
+constexpr float kFar = 1.0e30f;  // coordinate of ForceRule padding particles
 constexpr int kDppWaveRol1 = 0x134;
 
 // Lane l receives lane (l + 1) mod 64's value.
@@ -63,7 +72,7 @@ __host__ __device__ inline long long items_upto(long long m, int L) {
   return (long long)L * q * (q + 1) / 2 + r * (q + 1);
 }
 
-// First work item (= force-partial slot) of row ga.
+// First work item (= row-partial slot) of row ga.
 __host__ __device__ inline long long row_first(int ga, int ts, int L) {
   return items_upto(ts, L) - items_upto(ts - ga, L);
 }
@@ -75,83 +84,244 @@ __host__ __device__ inline long long pair_slot(int ga, int gb, int ts) {
 
 struct SymArgs {
   const float *px, *py, *pz;
-  float *vx, *vy, *vz;
-  float *nx, *ny, *nz;
-  float *fpart; // [items][3][ST] force partials
-  float *spart; // [ts (ts - 1) / 2][3][ST] reaction partials
+  const float *lxn[3], *lvn[3]; // adjoint: cotangents of the step's outputs
+  float *lx[3], *lv[3];         // adjoint: cotangents of its inputs
+  float *vx, *vy, *vz;          // velocity in/out (forward, fused adjoint)
+  float *nx, *ny, *nz;          // next positions (forward, fused adjoint)
+  float *fpart;                 // [items][ACC][ST] row partials
+  float *spart;                 // [ts (ts - 1) / 2][ACC][ST] reaction partials
   int n;
   int ts;    // super-tiles
   int chunk; // super-tiles per work item
   StepConsts c;
 };
 
-template <int U>
-struct Lane {
-  float x[U], y[U], z[U];    // targets
-  float fx[U], fy[U], fz[U]; // their forces
-  float sx[U], sy[U], sz[U]; // reaction sums of the current source set
+struct ForceRule {
+  static constexpr int SRC = 3; // x, y, z
+  static constexpr int ACC = 3; // F
+  static constexpr bool MASKED_TAIL = false;
+
+  static __device__ __forceinline__ void load(const SymArgs &a, int j,
+                                              float (&v)[SRC]) {
+    const bool ok = j < a.n;
+    v[0] = ok ? a.px[j] : kFar;
+    v[1] = ok ? a.py[j] : kFar;
+    v[2] = ok ? a.pz[j] : kFar;
+  }
+
+  // Source s on target t: a += r K; with SYM the reaction -r K goes to rs.
+  template <bool SYM, bool MASK>
+  static __device__ __forceinline__ void pair(const float (&s)[SRC],
+                                              const float (&t)[SRC],
+                                              float (&a)[ACC],
+                                              float (&rs)[ACC], float eps,
+                                              bool) {
+    const float rx = s[0] - t[0];
+    const float ry = s[1] - t[1];
+    const float rz = s[2] - t[2];
+    float d = __builtin_fmaf(rx, rx, eps);
+    d = __builtin_fmaf(ry, ry, d);
+    d = __builtin_fmaf(rz, rz, d);
+    const float K = __builtin_amdgcn_rsqf((d * d) * d);
+    a[0] = __builtin_fmaf(rx, K, a[0]);
+    a[1] = __builtin_fmaf(ry, K, a[1]);
+    a[2] = __builtin_fmaf(rz, K, a[2]);
+    if constexpr (SYM) {
+      rs[0] = __builtin_fmaf(-rx, K, rs[0]);
+      rs[1] = __builtin_fmaf(-ry, K, rs[1]);
+      rs[2] = __builtin_fmaf(-rz, K, rs[2]);
+    }
+  }
+
+  static __device__ __forceinline__ void finish(const SymArgs &a, int i,
+                                                const float (&f)[ACC]) {
+    integrate_ref(a.c, i, f[0], f[1], f[2], a.px, a.py, a.pz, a.vx, a.vy,
+                  a.vz, a.nx, a.ny, a.nz);
+  }
 };
 
-// 64 steps of one wave against one LDS source tile (3 x P floats at
-// tx/ty/tz).  With SYM the reaction sums rotate through the wave.
-template <int U, bool SYM>
-__device__ __forceinline__ void tile_pass(Lane<U> &L, const float4 *tx,
-                                          const float4 *ty, const float4 *tz,
-                                          int lane, float eps) {
+template <bool FUSED>
+struct AdjointRule {
+  static constexpr int SRC = 6; // x, y, z, g_x, g_y, g_z
+  static constexpr int ACC = FUSED ? 6 : 3; // sum M (g_j - g_k) [, F]
+  static constexpr bool MASKED_TAIL = true;
+
+  // g = lv' + dt lx' (the cotangent the pair sum is taken of)
+  static __device__ __forceinline__ void load(const SymArgs &a, int j,
+                                              float (&v)[SRC]) {
+    if (j < a.n) {
+      v[0] = a.px[j];
+      v[1] = a.py[j];
+      v[2] = a.pz[j];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        v[3 + c] = __builtin_fmaf(a.c.dt, a.lxn[c][j], a.lvn[c][j]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < SRC; ++c) v[c] = 0.0f;
+    }
+  }
+
+  template <bool SYM, bool MASK>
+  static __device__ __forceinline__ void pair(const float (&s)[SRC],
+                                              const float (&t)[SRC],
+                                              float (&a)[ACC],
+                                              float (&rs)[ACC], float eps,
+                                              bool valid) {
+    const float rx = s[0] - t[0];
+    const float ry = s[1] - t[1];
+    const float rz = s[2] - t[2];
+    float d = __builtin_fmaf(rx, rx, eps);
+    d = __builtin_fmaf(ry, ry, d);
+    d = __builtin_fmaf(rz, rz, d);
+    // K exactly as the forward kernels form it (rsqrt(d^3))
+    float K = __builtin_amdgcn_rsqf((d * d) * d);
+    const float q = __builtin_amdgcn_rcpf(d);
+    const float hx = s[3] - t[3];
+    const float hy = s[4] - t[4];
+    const float hz = s[5] - t[5];
+    float rh = rx * hx;
+    rh = __builtin_fmaf(ry, hy, rh);
+    rh = __builtin_fmaf(rz, hz, rh);
+    float t3 = (rh * q) * -3.0f;
+    if constexpr (MASK) {
+      K = valid ? K : 0.0f;
+      t3 = valid ? t3 : 0.0f;
+    }
+    const float ux = __builtin_fmaf(t3, rx, hx);
+    const float uy = __builtin_fmaf(t3, ry, hy);
+    const float uz = __builtin_fmaf(t3, rz, hz);
+    a[0] = __builtin_fmaf(K, ux, a[0]);
+    a[1] = __builtin_fmaf(K, uy, a[1]);
+    a[2] = __builtin_fmaf(K, uz, a[2]);
+    if constexpr (SYM) {
+      rs[0] = __builtin_fmaf(-K, ux, rs[0]);
+      rs[1] = __builtin_fmaf(-K, uy, rs[1]);
+      rs[2] = __builtin_fmaf(-K, uz, rs[2]);
+    }
+    if constexpr (FUSED) {
+      a[3] = __builtin_fmaf(rx, K, a[3]);
+      a[4] = __builtin_fmaf(ry, K, a[4]);
+      a[5] = __builtin_fmaf(rz, K, a[5]);
+      if constexpr (SYM) {
+        rs[3] = __builtin_fmaf(-rx, K, rs[3]);
+        rs[4] = __builtin_fmaf(-ry, K, rs[4]);
+        rs[5] = __builtin_fmaf(-rz, K, rs[5]);
+      }
+    }
+  }
+
+  // lx = lx' + dt G sum,  lv = damping g; fused: the forward step too.
+  static __device__ __forceinline__ void finish(const SymArgs &a, int i,
+                                                const float (&f)[ACC]) {
+    const float cG = a.c.dt * a.c.G;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float g = __builtin_fmaf(a.c.dt, a.lxn[c][i], a.lvn[c][i]);
+      a.lx[c][i] = __builtin_fmaf(cG, f[c], a.lxn[c][i]);
+      a.lv[c][i] = a.c.damping * g;
+    }
+    if constexpr (FUSED)
+      integrate_ref(a.c, i, f[3], f[4], f[5], a.px, a.py, a.pz, a.vx, a.vy,
+                    a.vz, a.nx, a.ny, a.nz);
+  }
+};
+
+template <class R, int U>
+struct Lane {
+  float t[U][R::SRC];  // target inputs
+  float a[U][R::ACC];  // target sums
+  float rs[U][R::ACC]; // reaction sums of the current source set
+};
+
+// 64 steps of one wave against source tile j of the LDS super-tile
+// (src: R::SRC arrays of ST floats).  With SYM the reaction sums rotate
+// through the wave.  With MASK, pairs with a padding particle contribute
+// nothing (vt: target validity; sources are valid below index n).
+template <class R, int U, bool SYM, bool MASK>
+__device__ __forceinline__ void tile_pass(Lane<R, U> &L, const float *src,
+                                          int j, int lane, float eps,
+                                          const bool (&vt)[U], int src0,
+                                          int n) {
+  constexpr int P = 64 * U;
+  constexpr int ST = kWaves * P;
   constexpr int Q = U / 4;
   for (int s = 0; s < 64; ++s) {
     const int m = (lane + s) & 63;
-    float xs[U], ys[U], zs[U];
+    float sv[U][R::SRC];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const float4 X = tx[m * Q + q], Y = ty[m * Q + q], Z = tz[m * Q + q];
-      xs[4 * q + 0] = X.x; xs[4 * q + 1] = X.y;
-      xs[4 * q + 2] = X.z; xs[4 * q + 3] = X.w;
-      ys[4 * q + 0] = Y.x; ys[4 * q + 1] = Y.y;
-      ys[4 * q + 2] = Y.z; ys[4 * q + 3] = Y.w;
-      zs[4 * q + 0] = Z.x; zs[4 * q + 1] = Z.y;
-      zs[4 * q + 2] = Z.z; zs[4 * q + 3] = Z.w;
+    for (int c = 0; c < R::SRC; ++c) {
+      const float4 *p4 = reinterpret_cast<const float4 *>(src + c * ST + j * P);
+#pragma unroll
+      for (int qq = 0; qq < Q; ++qq) {
+        const float4 v = p4[m * Q + qq];
+        sv[4 * qq + 0][c] = v.x;
+        sv[4 * qq + 1][c] = v.y;
+        sv[4 * qq + 2][c] = v.z;
+        sv[4 * qq + 3][c] = v.w;
+      }
     }
 #pragma unroll
     for (int q = 0; q < U; ++q) {
+      const bool vs = !MASK || (src0 + m * U + q < n);
 #pragma unroll
-      for (int k = 0; k < U; ++k) {
-        const float rx = xs[q] - L.x[k];
-        const float ry = ys[q] - L.y[k];
-        const float rz = zs[q] - L.z[k];
-        float d = __builtin_fmaf(rx, rx, eps);
-        d = __builtin_fmaf(ry, ry, d);
-        d = __builtin_fmaf(rz, rz, d);
-        const float inv = __builtin_amdgcn_rsqf((d * d) * d);
-        L.fx[k] = __builtin_fmaf(rx, inv, L.fx[k]);
-        L.fy[k] = __builtin_fmaf(ry, inv, L.fy[k]);
-        L.fz[k] = __builtin_fmaf(rz, inv, L.fz[k]);
-        if constexpr (SYM) {
-          L.sx[q] = __builtin_fmaf(-rx, inv, L.sx[q]);
-          L.sy[q] = __builtin_fmaf(-ry, inv, L.sy[q]);
-          L.sz[q] = __builtin_fmaf(-rz, inv, L.sz[q]);
-        }
-      }
+      for (int k = 0; k < U; ++k)
+        R::template pair<SYM, MASK>(sv[q], L.t[k], L.a[k], L.rs[q], eps,
+                                    vs && vt[k]);
     }
     if constexpr (SYM) {
 #pragma unroll
-      for (int q = 0; q < U; ++q) {
-        L.sx[q] = wave_rol1(L.sx[q]);
-        L.sy[q] = wave_rol1(L.sy[q]);
-        L.sz[q] = wave_rol1(L.sz[q]);
-      }
+      for (int q = 0; q < U; ++q)
+#pragma unroll
+        for (int c = 0; c < R::ACC; ++c) L.rs[q][c] = wave_rol1(L.rs[q][c]);
     }
   }
 }
 
-template <int U>
+template <class R, int U, bool MASK>
+__device__ __forceinline__ void super_pair(Lane<R, U> &L, const float *src,
+                                           float *sacc, bool diag, int w,
+                                           int lane, float eps,
+                                           const bool (&vt)[U], int gb0src,
+                                           int n) {
+  constexpr int P = 64 * U;
+  constexpr int ST = kWaves * P;
+  if (diag) {
+#pragma unroll 1
+    for (int b = 0; b < kWaves; ++b) {
+      const int j = (w + b) & (kWaves - 1);
+      tile_pass<R, U, false, MASK>(L, src, j, lane, eps, vt, gb0src + j * P,
+                                   n);
+    }
+    return;
+  }
+#pragma unroll 1
+  for (int b = 0; b < kWaves; ++b) {
+    const int j = (w + b) & (kWaves - 1);
+    // tile j's sums start at zero with its first visitor (b = 0, w = j)
+    const int o = j * P + lane * U;
+#pragma unroll
+    for (int q = 0; q < U; ++q)
+#pragma unroll
+      for (int c = 0; c < R::ACC; ++c)
+        L.rs[q][c] = b ? sacc[c * ST + o + q] : 0.0f;
+    tile_pass<R, U, true, MASK>(L, src, j, lane, eps, vt, gb0src + j * P, n);
+#pragma unroll
+    for (int q = 0; q < U; ++q)
+#pragma unroll
+      for (int c = 0; c < R::ACC; ++c) sacc[c * ST + o + q] = L.rs[q][c];
+    __syncthreads(); // tile j passes to the next wave
+  }
+}
+
+template <class R, int U>
 __global__ __launch_bounds__(256) void sym_kernel(SymArgs a) {
   constexpr int P = 64 * U;
   constexpr int ST = kWaves * P;
-  __shared__ float4 src[3][ST / 4];  // source super-tile, SoA
-  __shared__ float4 sacc[3][ST / 4]; // reaction sums of its four tiles
-  float *fsrc = reinterpret_cast<float *>(src);
-  float *fsacc = reinterpret_cast<float *>(sacc);
+  __shared__ float4 src4[R::SRC][ST / 4];  // source super-tile, SoA
+  __shared__ float4 sacc4[R::ACC][ST / 4]; // reaction sums of its 4 tiles
+  float *src = reinterpret_cast<float *>(src4);
+  float *sacc = reinterpret_cast<float *>(sacc4);
   const int tid = threadIdx.x;
   const int w = tid >> 6, lane = tid & 63;
   const int n = a.n, ts = a.ts, L = a.chunk;
@@ -169,79 +339,56 @@ __global__ __launch_bounds__(256) void sym_kernel(SymArgs a) {
   const int c = (int)(it - row_first(ga, ts, L));
   const int gb0 = ga + c * L;
   const int gb1 = min(gb0 + L, ts);
+  const bool ragged = (n % ST) != 0;
 
-  Lane<U> T;
+  Lane<R, U> T;
+  bool vt[U];
   const int t0 = ga * ST + w * P + lane * U;
 #pragma unroll
   for (int k = 0; k < U; ++k) {
-    const int i = t0 + k;
-    const bool ok = i < n;
-    T.x[k] = ok ? a.px[i] : kFar;
-    T.y[k] = ok ? a.py[i] : kFar;
-    T.z[k] = ok ? a.pz[i] : kFar;
-    T.fx[k] = T.fy[k] = T.fz[k] = 0.0f;
+    R::load(a, t0 + k, T.t[k]);
+    vt[k] = t0 + k < n;
+#pragma unroll
+    for (int e = 0; e < R::ACC; ++e) T.a[k][e] = 0.0f;
   }
 
   for (int gb = gb0; gb < gb1; ++gb) {
     __syncthreads(); // the previous super-tile and its sums are written out
 #pragma unroll
     for (int l = 0; l < ST / 256; ++l) {
-      const int j = gb * ST + l * 256 + tid;
-      const bool ok = j < n;
-      fsrc[0 * ST + l * 256 + tid] = ok ? a.px[j] : kFar;
-      fsrc[1 * ST + l * 256 + tid] = ok ? a.py[j] : kFar;
-      fsrc[2 * ST + l * 256 + tid] = ok ? a.pz[j] : kFar;
+      float v[R::SRC];
+      R::load(a, gb * ST + l * 256 + tid, v);
+#pragma unroll
+      for (int e = 0; e < R::SRC; ++e) src[e * ST + l * 256 + tid] = v[e];
     }
     __syncthreads();
 
-    if (gb == ga) {
-#pragma unroll 1
-      for (int b = 0; b < kWaves; ++b) {
-        const int j = (w + b) & (kWaves - 1);
-        tile_pass<U, false>(T, src[0] + j * (P / 4), src[1] + j * (P / 4),
-                            src[2] + j * (P / 4), lane, a.c.eps);
-      }
-      continue;
-    }
-
-#pragma unroll 1
-    for (int b = 0; b < kWaves; ++b) {
-      const int j = (w + b) & (kWaves - 1);
-      // tile j's sums start at zero with its first visitor (b = 0, w = j)
-      const int o = j * P + lane * U;
+    // only the last super-tile holds padding particles (gb >= ga)
+    bool masked = false;
+    if constexpr (R::MASKED_TAIL) masked = ragged && gb == ts - 1;
+    if (masked)
+      super_pair<R, U, R::MASKED_TAIL>(T, src, sacc, gb == ga, w, lane,
+                                       a.c.eps, vt, gb * ST, n);
+    else
+      super_pair<R, U, false>(T, src, sacc, gb == ga, w, lane, a.c.eps, vt,
+                              gb * ST, n);
+    if (gb == ga) continue;
+    float *sp = a.spart + (size_t)pair_slot(ga, gb, ts) * (R::ACC * ST);
 #pragma unroll
-      for (int q = 0; q < U; ++q) {
-        T.sx[q] = b ? fsacc[0 * ST + o + q] : 0.0f;
-        T.sy[q] = b ? fsacc[1 * ST + o + q] : 0.0f;
-        T.sz[q] = b ? fsacc[2 * ST + o + q] : 0.0f;
-      }
-      tile_pass<U, true>(T, src[0] + j * (P / 4), src[1] + j * (P / 4),
-                         src[2] + j * (P / 4), lane, a.c.eps);
-#pragma unroll
-      for (int q = 0; q < U; ++q) {
-        fsacc[0 * ST + o + q] = T.sx[q];
-        fsacc[1 * ST + o + q] = T.sy[q];
-        fsacc[2 * ST + o + q] = T.sz[q];
-      }
-      __syncthreads(); // tile j passes to the next wave
-    }
-    float *sp = a.spart + (size_t)pair_slot(ga, gb, ts) * (3 * ST);
-#pragma unroll
-    for (int l = 0; l < 3 * ST / 256; ++l) sp[l * 256 + tid] = fsacc[l * 256 + tid];
+    for (int l = 0; l < R::ACC * ST / 256; ++l)
+      sp[l * 256 + tid] = sacc[l * 256 + tid];
   }
 
-  float *fp = a.fpart + (size_t)it * (3 * ST) + w * P + lane * U;
+  float *fp = a.fpart + (size_t)it * (R::ACC * ST) + w * P + lane * U;
 #pragma unroll
-  for (int k = 0; k < U; ++k) {
-    fp[0 * ST + k] = T.fx[k];
-    fp[1 * ST + k] = T.fy[k];
-    fp[2 * ST + k] = T.fz[k];
-  }
+  for (int k = 0; k < U; ++k)
+#pragma unroll
+    for (int e = 0; e < R::ACC; ++e) fp[e * ST + k] = T.a[k][e];
 }
 
-// Per particle: force partials of its row (chunk order), then reaction
-// partials of its column (ga order), then the reference's integrator.
-template <int U>
+// Per particle: row partials (chunk order), then the reaction partials of
+// its column (ga order), then the rule's epilogue.
+template <class R, int U>
 __global__ __launch_bounds__(256) void sym_finalize(SymArgs a) {
   constexpr int ST = kWaves * 64 * U;
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -250,21 +397,20 @@ __global__ __launch_bounds__(256) void sym_finalize(SymArgs a) {
   const int ts = a.ts, L = a.chunk;
   const long long first = row_first(g, ts, L);
   const int nc = (ts - g + L - 1) / L;
-  float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+  float f[R::ACC];
+#pragma unroll
+  for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
   for (int c = 0; c < nc; ++c) {
-    const float *p = a.fpart + (size_t)(first + c) * (3 * ST) + o;
-    f0 = f0 + p[0];
-    f1 = f1 + p[ST];
-    f2 = f2 + p[2 * ST];
+    const float *p = a.fpart + (size_t)(first + c) * (R::ACC * ST) + o;
+#pragma unroll
+    for (int e = 0; e < R::ACC; ++e) f[e] = f[e] + p[e * ST];
   }
   for (int ga = 0; ga < g; ++ga) {
-    const float *p = a.spart + (size_t)pair_slot(ga, g, ts) * (3 * ST) + o;
-    f0 = f0 + p[0];
-    f1 = f1 + p[ST];
-    f2 = f2 + p[2 * ST];
+    const float *p = a.spart + (size_t)pair_slot(ga, g, ts) * (R::ACC * ST) + o;
+#pragma unroll
+    for (int e = 0; e < R::ACC; ++e) f[e] = f[e] + p[e * ST];
   }
-  integrate_ref(a.c, i, f0, f1, f2, a.px, a.py, a.pz, a.vx, a.vy, a.vz, a.nx,
-                a.ny, a.nz);
+  R::finish(a, i, f);
 }
 
 using KernelFn = void (*)(SymArgs);
@@ -274,36 +420,22 @@ struct SymShape {
   KernelFn pairs, finalize;
 };
 
-const SymShape kSymShapes[] = {
-    {4, sym_kernel<4>, sym_finalize<4>},
-    {8, sym_kernel<8>, sym_finalize<8>},
-};
-
+template <class R>
 const SymShape *find_shape(int unroll) {
-  for (const SymShape &s : kSymShapes)
+  static const SymShape shapes[] = {
+      {4, sym_kernel<R, 4>, sym_finalize<R, 4>},
+      {8, sym_kernel<R, 8>, sym_finalize<R, 8>},
+  };
+  for (const SymShape &s : shapes)
     if (s.unroll == unroll) return &s;
   return nullptr;
 }
 
-} // namespace
-
-size_t sym_scratch_bytes(int unroll, int chunk, size_t n) {
-  if (n == 0 || n > kSymMaxParticles || chunk < 1) return 0;
-  const long long st = (long long)kWaves * 64 * unroll;
-  const long long ts = ((long long)n + st - 1) / st;
-  const long long slots = items_upto(ts, chunk) + ts * (ts - 1) / 2;
-  return sizeof(float) * (size_t)slots * 3 * (size_t)st;
-}
-
-int launch_forward_sym(const StepConsts &c, int unroll, int chunk,
-                       const float *px, const float *py, const float *pz,
-                       float *vx, float *vy, float *vz, float *nx, float *ny,
-                       float *nz, int n, float *scratch, hipStream_t stream) {
-  if (n <= 0) return NB_OK;
+int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
+               float *scratch, int acc, hipStream_t stream) {
   if ((size_t)n > kSymMaxParticles)
     return set_error(NB_ERR_UNSUPPORTED,
                      "symmetric launch shapes take at most 2^20 particles");
-  const SymShape *sh = find_shape(unroll);
   if (!sh || chunk < 1)
     return set_error(NB_ERR_UNSUPPORTED, "no symmetric kernel for this shape");
   if (!scratch)
@@ -311,8 +443,39 @@ int launch_forward_sym(const StepConsts &c, int unroll, int chunk,
   const int st = kWaves * 64 * unroll;
   const int ts = (n + st - 1) / st;
   const long long items = items_upto(ts, chunk);
+  a.fpart = scratch;
+  a.spart = scratch + (size_t)items * acc * st;
+  a.n = n;
+  a.ts = ts;
+  a.chunk = chunk;
+  void *args[] = {&a};
+  int r = hip_check(hipLaunchKernel(reinterpret_cast<const void *>(sh->pairs),
+                                    dim3((unsigned)items), dim3(256), args, 0,
+                                    stream),
+                    "symmetric pair launch");
+  if (r) return r;
+  return hip_check(
+      hipLaunchKernel(reinterpret_cast<const void *>(sh->finalize),
+                      dim3((n + 255) / 256), dim3(256), args, 0, stream),
+      "symmetric finalize launch");
+}
 
-  SymArgs a;
+} // namespace
+
+size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc) {
+  if (n == 0 || n > kSymMaxParticles || chunk < 1) return 0;
+  const long long st = (long long)kWaves * 64 * unroll;
+  const long long ts = ((long long)n + st - 1) / st;
+  const long long slots = items_upto(ts, chunk) + ts * (ts - 1) / 2;
+  return sizeof(float) * (size_t)slots * acc * (size_t)st;
+}
+
+int launch_forward_sym(const StepConsts &c, int unroll, int chunk,
+                       const float *px, const float *py, const float *pz,
+                       float *vx, float *vy, float *vz, float *nx, float *ny,
+                       float *nz, int n, float *scratch, hipStream_t stream) {
+  if (n <= 0) return NB_OK;
+  SymArgs a = {};
   a.px = px;
   a.py = py;
   a.pz = pz;
@@ -322,22 +485,41 @@ int launch_forward_sym(const StepConsts &c, int unroll, int chunk,
   a.nx = nx;
   a.ny = ny;
   a.nz = nz;
-  a.fpart = scratch;
-  a.spart = scratch + (size_t)items * 3 * st;
-  a.n = n;
-  a.ts = ts;
-  a.chunk = chunk;
   a.c = c;
-  void *args[] = {&a};
-  int r = hip_check(hipLaunchKernel(reinterpret_cast<const void *>(sh->pairs),
-                                    dim3((unsigned)items), dim3(256), args, 0,
-                                    stream),
-                    "symmetric forward launch");
-  if (r) return r;
-  return hip_check(
-      hipLaunchKernel(reinterpret_cast<const void *>(sh->finalize),
-                      dim3((n + 255) / 256), dim3(256), args, 0, stream),
-      "symmetric forward finalize launch");
+  return launch_sym(find_shape<ForceRule>(unroll), a, unroll, chunk, n,
+                    scratch, ForceRule::ACC, stream);
+}
+
+int launch_adjoint_sym(const StepConsts &c, bool fused, int unroll, int chunk,
+                       const float *const pos[3], const float *const lxn[3],
+                       const float *const lvn[3], float *const lx[3],
+                       float *const lv[3], float *const vel[3],
+                       float *const nxt[3], int n, float *scratch,
+                       hipStream_t stream) {
+  if (n <= 0) return NB_OK;
+  SymArgs a = {};
+  a.c = c;
+  a.px = pos[0];
+  a.py = pos[1];
+  a.pz = pos[2];
+  for (int d = 0; d < 3; ++d) {
+    a.lxn[d] = lxn[d];
+    a.lvn[d] = lvn[d];
+    a.lx[d] = lx[d];
+    a.lv[d] = lv[d];
+  }
+  if (fused) {
+    a.vx = vel[0];
+    a.vy = vel[1];
+    a.vz = vel[2];
+    a.nx = nxt[0];
+    a.ny = nxt[1];
+    a.nz = nxt[2];
+    return launch_sym(find_shape<AdjointRule<true>>(unroll), a, unroll, chunk,
+                      n, scratch, AdjointRule<true>::ACC, stream);
+  }
+  return launch_sym(find_shape<AdjointRule<false>>(unroll), a, unroll, chunk,
+                    n, scratch, AdjointRule<false>::ACC, stream);
 }
 
 } // namespace nb

@@ -58,7 +58,7 @@ int adjoint_common(const nb_params *p, const nb_vec3 *pos, const nb_vec3 *vel,
       overlaps(lv, lvn) || overlaps(lx, lv) ||
       (fused && (overlaps(nxt, pos) || overlaps(vel, pos))))
     return set_error(NB_ERR_INVALID_ARG, "outputs must not alias inputs");
-  if (cfg < 0) cfg = adjoint_default_config(n);
+  if (cfg < 0) cfg = adjoint_default_config(n, !self_term_vanishes(p->dist_eps));
   float *scratch = nullptr;
   if (const size_t bytes = adjoint_scratch_bytes(cfg, n)) {
     scratch = device_scratch(bytes, &st);
@@ -96,10 +96,12 @@ TrajLayout traj_layout(size_t n, int every) {
   // the forward shape depends on the pair mode: size for both candidates
   const int cf = forward_default_config(n, MODE_PLAIN);
   const int cs = forward_default_config(n, MODE_PLAIN_SKIP);
-  const int ca = adjoint_default_config(n);
+  const int ca = adjoint_default_config(n, false);
+  const int ck = adjoint_default_config(n, true);
   const size_t sb = std::max({forward_scratch_bytes(cf, n),
                               forward_scratch_bytes(cs, n),
-                              adjoint_scratch_bytes(ca, n)});
+                              adjoint_scratch_bytes(ca, n),
+                              adjoint_scratch_bytes(ck, n)});
   L.total = L.scratch + (sb + sizeof(float) - 1) / sizeof(float);
   return L;
 }
@@ -161,6 +163,14 @@ int nb_adjoint_config_info(int cfg, int *block, int *unroll, int *tile,
   if (tile) *tile = c.tile;
   if (split) *split = c.split;
   return NB_OK;
+}
+
+int nb_adjoint_config_scheme(int cfg) {
+  if (cfg < 0 || cfg >= adjoint_config_count()) {
+    set_error(NB_ERR_INVALID_ARG, "adjoint config out of range");
+    return -1;
+  }
+  return adjoint_config(cfg).scheme;
 }
 
 size_t nb_traj_checkpoint_floats(size_t n, int steps, int every) {
@@ -226,7 +236,7 @@ int nb_traj_backward_dev(const nb_params *p, const float *checkpoints,
   auto replay = [&](int m) { return workspace + L.replay + (size_t)m * n3; };
   const StepConsts c = consts(p);
   const bool skip = !self_term_vanishes(p->dist_eps);
-  const int cfg_a = adjoint_default_config(n);
+  const int cfg_a = adjoint_default_config(n, skip);
 
   if ((st = copy_d2d(lam[0], lam_final, 2 * n3, s))) return st;
   int cur = 0;

@@ -126,6 +126,7 @@ def load() -> ctypes.CDLL:
          [POINTER(NbParams), V3, V3, V3, V3, V3, V3, V3, c_size_t, c_int, vp])
     _sig(lib, "nb_adjoint_config_count", c_int, [])
     _sig(lib, "nb_adjoint_config_info", c_int, [c_int] + [POINTER(c_int)] * 4)
+    _sig(lib, "nb_adjoint_config_scheme", c_int, [c_int])
     _sig(lib, "nb_traj_checkpoint_floats", c_size_t, [c_size_t, c_int, c_int])
     _sig(lib, "nb_traj_workspace_floats", c_size_t, [c_size_t, c_int])
     _sig(lib, "nb_traj_forward_dev", c_int,

@@ -13,7 +13,7 @@ from __future__ import annotations
 
 import ctypes
 
-from ._lib import NbParams, NbVec3, check, load
+from ._lib import NB_ERR_INVALID_ARG, NbParams, NbVec3, check, load
 
 
 def _vec3(t) -> NbVec3:
@@ -55,9 +55,16 @@ def adjoint_config_count() -> int:
 
 
 def adjoint_config_info(cfg: int) -> dict:
+    """Adjoint launch shape `cfg` (fields as forward_config_info's)."""
+    lib = load()
     vals = [ctypes.c_int() for _ in range(4)]
-    check(load().nb_adjoint_config_info(cfg, *[ctypes.byref(v) for v in vals]))
-    return dict(zip(("block", "unroll", "tile", "split"), (v.value for v in vals)))
+    check(lib.nb_adjoint_config_info(cfg, *[ctypes.byref(v) for v in vals]))
+    info = dict(zip(("block", "unroll", "tile", "split"), (v.value for v in vals)))
+    scheme = int(lib.nb_adjoint_config_scheme(cfg))
+    if scheme < 0:
+        check(NB_ERR_INVALID_ARG)
+    info["scheme"] = {0: "targets", 1: "symmetric"}[scheme]
+    return info
 
 
 def _ptr(t) -> ctypes.c_void_p:

@@ -5,7 +5,8 @@
 
 precision: N particles, half a Plummer sphere (scale radius 1, virial
   velocities for the simulation's own G and unit masses, centred on the disk
-  mid-plane) and half the reference disk, seed 7.  For plain fp32 and
+  mid-plane) and half the reference disk, seed 7.  For plain fp32 (the
+  library's symmetric scheme and the target-sharded one) and
   Kahan-compensated fp32 force accumulation: force error percentiles against
   the fp64 GPU oracle on a random 4096-particle sample, throughput over
   `steps` steps, and the fp64 total energy and momentum before and after.
@@ -83,10 +84,16 @@ def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4
                            "value": sample * n / t_f64 / 1e9, "unit": "Ginteractions/s"}}
     e0 = nb.energy_momentum(params, P0, V0)
     out["energy_initial"] = e0
-    for name, acc in (("fp32", nb.NB_ACC_PLAIN), ("fp32_kahan", nb.NB_ACC_KAHAN)):
+    targets = next(c for c in range(nb.forward_config_count())
+                   if nb.forward_config_info(c) == {"block": 256, "unroll": 4, "tile": 1024,
+                                                    "split": 8, "scheme": "targets"})
+    modes = (("fp32", nb.NB_ACC_PLAIN, -1),            # library default (symmetric)
+             ("fp32_targets", nb.NB_ACC_PLAIN, targets),
+             ("fp32_kahan", nb.NB_ACC_KAHAN, -1))
+    for name, acc, cfg in modes:
         V = torch.zeros_like(P0)
         N = torch.empty_like(P0)
-        nb.forward_step(unit, list(P0), list(V), list(N), acc=acc)
+        nb.forward_step(unit, list(P0), list(V), list(N), acc=acc, cfg=cfg)
         torch.cuda.synchronize()
         F = V[:, idx.long()].double().cpu().numpy()
         rel = np.linalg.norm(F - f64, axis=0) / norm64
@@ -94,6 +101,8 @@ def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4
         sim = nb.Simulator(nb.default_params(num_particles=n, sim_iterations_per_frame=steps),
                            pos, vel)
         sim.set_accumulation(acc)
+        sim.set_config(cfg)
+        launch = nb.forward_config_info(sim.config)
         sim.step_sim()
         ms = sim.last_kernel_ms / steps
         xp = torch.from_numpy(np.stack(sim.positions())).to(dev)
@@ -104,6 +113,7 @@ def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4
         pscale = float(np.abs(np.stack(vel).astype(np.float64)).sum())
         dp = np.array(e1["momentum"]) - damp * np.array(e0["momentum"])
         out[name] = {
+            "launch": launch,
             "force_rel_err_vs_fp64": _percentiles(rel),
             "ms_per_step": ms,
             "value": n * float(n) / (ms * 1e-3) / 1e9,

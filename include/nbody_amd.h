@@ -234,6 +234,9 @@ int nb_fwd_adjoint_step_dev(const nb_params *p, const nb_vec3 *pos,
 int nb_adjoint_config_count(void);
 int nb_adjoint_config_info(int cfg, int *block, int *unroll, int *tile,
                            int *split);
+/* Scheme of an adjoint launch shape (NB_SCHEME_*; the symmetric shapes need
+ * dist_eps^3 a normal float and n <= 2^20), or -1 for a bad index. */
+int nb_adjoint_config_scheme(int cfg);
 
 /* ---- differentiable trajectory with checkpoints (extension) ------------- */
 /* States are [6][n] float blocks: x, y, z, vx, vy, vz (n floats each).

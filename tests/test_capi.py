@@ -178,6 +178,18 @@ def test_forward_configs_are_described(nb):
     assert lib.nb_forward_config_scheme(n) == -1 and lib.nb_forward_config_scheme(-1) == -1
 
 
+def test_adjoint_configs_are_described(nb):
+    n = nb.adjoint_config_count()
+    schemes = [nb.adjoint_config_info(c)["scheme"] for c in range(n)]
+    assert schemes[0] == "targets" and "symmetric" in schemes
+    for c in range(n):  # every symmetric adjoint shape has a forward twin
+        info = nb.adjoint_config_info(c)
+        if info["scheme"] == "symmetric":
+            assert any(nb.forward_config_info(f) == info
+                       for f in range(nb.forward_config_count()))
+    assert nb.load().nb_adjoint_config_scheme(n) == -1
+
+
 def test_driver_rejects_bad_method(nb):
     res = subprocess.run([str(nb.DRIVER_PATH), "1", "1", "1", "1", "1", "1", "1", "64", "X"],
                          capture_output=True, text=True, timeout=60)

@@ -29,10 +29,29 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def test_adjoint_pair_sum_vs_fp64(nb, orc, gpu):
+def sym_adjoint_configs(nb):
+    return [c for c in range(nb.adjoint_config_count())
+            if nb.adjoint_config_info(c)["scheme"] == "symmetric"]
+
+
+def matching_forward_config(nb, acfg):
+    """The forward shape with the adjoint shape's decomposition (same scheme,
+    unroll, tile and split): the fused step's forward part equals its bits."""
+    a = nb.adjoint_config_info(acfg)
+    for c in range(nb.forward_config_count()):
+        if nb.forward_config_info(c) == a:
+            return c
+    raise AssertionError(f"no forward shape like {a}")
+
+
+@pytest.mark.parametrize("cfg", ["default", "sym0", "sym1", "sym2"])
+def test_adjoint_pair_sum_vs_fp64(nb, orc, gpu, cfg):
     """With dt = G = 1 and lam_x' = 0 the kernel's lam_x is the pair sum
-    J^T g itself; compare it per particle with the fp64 sum."""
+    J^T g itself; compare it per particle with the fp64 sum (3000 particles:
+    a partial last tile for every shape)."""
     import torch
+
+    cfg = -1 if cfg == "default" else sym_adjoint_configs(nb)[int(cfg[3:])]
 
     n = 3000
     pos, _ = nb.plummer_ic(n, seed=21, g_mass=1.0)
@@ -44,7 +63,7 @@ def test_adjoint_pair_sum_vs_fp64(nb, orc, gpu):
     lvn = torch.from_numpy(g).to(gpu)
     lx = torch.empty(3, n, device=gpu)
     lv = torch.empty(3, n, device=gpu)
-    nb.adjoint_step(params, P, lxn, lvn, lx, lv)
+    nb.adjoint_step(params, P, lxn, lvn, lx, lv, cfg=cfg)
     torch.cuda.synchronize()
     p = orc.params(dt=1.0, G=1.0, damping=0.5)
     ref_lx, ref_lv = orc.adjoint_step_f64(p, pos, np.zeros((3, n)), g.astype(np.float64))
@@ -56,10 +75,12 @@ def test_adjoint_pair_sum_vs_fp64(nb, orc, gpu):
     assert np.array_equal(lv.cpu().numpy(), np.float32(0.5) * g)
 
 
-def test_adjoint_step_default_params(nb, orc, gpu):
+@pytest.mark.parametrize("n", [4096, 33000])
+def test_adjoint_step_default_params(nb, orc, gpu, n):
+    """SimParam defaults, the library's shape (target-sharded at 4096,
+    symmetric at 33000), against the fp64 adjoint."""
     import torch
 
-    n = 4096
     pos, _ = nb.disk_ic(n)
     rng = np.random.default_rng(3)
     lxn = rng.normal(size=(3, n)).astype(np.float32)
@@ -79,11 +100,15 @@ def test_adjoint_step_default_params(nb, orc, gpu):
     assert rel(outs[1].cpu().numpy(), np.stack(ref_lv)) < 1e-6
 
 
-@pytest.mark.parametrize("n", [1, 2, 257, 1500])
-def test_fused_step_equals_separate(nb, gpu, n):
+@pytest.mark.parametrize("shape", ["targets", "symmetric"])
+@pytest.mark.parametrize("n", [1, 2, 257, 1500, 5000])
+def test_fused_step_equals_separate(nb, gpu, n, shape):
     """Forward part bit-equal to the forward kernel, adjoint part bit-equal
-    to the adjoint kernel (same launch shape, same source order)."""
+    to the adjoint kernel (same decomposition, same summation order)."""
     import torch
+
+    acfg = 0 if shape == "targets" else sym_adjoint_configs(nb)[0]
+    fcfg = matching_forward_config(nb, acfg)
 
     pos, vel = nb.plummer_ic(n, seed=n, g_mass=1.0)
     params = nb.default_params(num_particles=n, G=1.0 / n)
@@ -96,9 +121,9 @@ def test_fused_step_equals_separate(nb, gpu, n):
     N1 = torch.empty_like(P)
     N2 = torch.empty_like(P)
     lx1, lv1, lx2, lv2 = [torch.empty(3, n, device=gpu) for _ in range(4)]
-    nb.forward_step(params, list(P), list(V1), list(N1), cfg=0)
-    nb.adjoint_step(params, P, lxn, lvn, lx1, lv1, cfg=0)
-    nb.fwd_adjoint_step(params, P, V2, N2, lxn, lvn, lx2, lv2, cfg=0)
+    nb.forward_step(params, list(P), list(V1), list(N1), cfg=fcfg)
+    nb.adjoint_step(params, P, lxn, lvn, lx1, lv1, cfg=acfg)
+    nb.fwd_adjoint_step(params, P, V2, N2, lxn, lvn, lx2, lv2, cfg=acfg)
     torch.cuda.synchronize()
     for a, b in ((V1, V2), (N1, N2), (lx1, lx2), (lv1, lv2)):
         assert torch.equal(a, b)
