@@ -59,11 +59,13 @@ constexpr int kWaves = 4;        // waves per workgroup = tiles per super-tile
 constexpr float kFar = 1.0e30f;  // coordinate of ForceRule padding particles
 constexpr int kDppWaveRol1 = 0x134;
 
-// Lane l receives lane (l + 1) mod 64's value.
+// Lane l receives lane (l + 1) mod 64's value.  Every lane is written (full
+// row and bank masks), so the "old" operand is the source itself: with a
+// constant there the compiler materialises it with an extra v_mov per call.
 __device__ __forceinline__ float wave_rol1(float v) {
+  const int b = __builtin_bit_cast(int, v);
   return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
-                                         kDppWaveRol1, 0xf, 0xf, false));
+      float, __builtin_amdgcn_update_dpp(b, b, kDppWaveRol1, 0xf, 0xf, false));
 }
 
 // sum_{l=1..m} ceil(l / L): work items of the rows of lengths 1..m.
@@ -227,6 +229,19 @@ struct AdjointRule {
   }
 };
 
+// LDS position of super-tile particle o (particle order o = j P + m U + q:
+// tile j, source set m, member q).  Within a tile, float4 slot qq * 64 + m
+// holds members 4 qq .. 4 qq + 3 of set m, so the 64 lanes of a step (sets
+// (l + s) mod 64, one quad each) read one contiguous 1 KiB block: no bank
+// conflicts.
+template <int U>
+__device__ __forceinline__ int lds_slot(int o) {
+  constexpr int P = 64 * U;
+  const int j = o / P, r = o % P;
+  const int m = r / U, q = r % U;
+  return j * P + (q >> 2) * 256 + m * 4 + (q & 3);
+}
+
 template <class R, int U>
 struct Lane {
   float t[U][R::SRC];  // target inputs
@@ -235,7 +250,7 @@ struct Lane {
 };
 
 // 64 steps of one wave against source tile j of the LDS super-tile
-// (src: R::SRC arrays of ST floats).  With SYM the reaction sums rotate
+// (src: R::SRC arrays of ST floats in lds_slot order).  With SYM the reaction sums rotate
 // through the wave.  With MASK, pairs with a padding particle contribute
 // nothing (vt: target validity; sources are valid below index n).
 template <class R, int U, bool SYM, bool MASK>
@@ -254,7 +269,7 @@ __device__ __forceinline__ void tile_pass(Lane<R, U> &L, const float *src,
       const float4 *p4 = reinterpret_cast<const float4 *>(src + c * ST + j * P);
 #pragma unroll
       for (int qq = 0; qq < Q; ++qq) {
-        const float4 v = p4[m * Q + qq];
+        const float4 v = p4[qq * 64 + m];
         sv[4 * qq + 0][c] = v.x;
         sv[4 * qq + 1][c] = v.y;
         sv[4 * qq + 2][c] = v.z;
@@ -358,8 +373,9 @@ __global__ __launch_bounds__(256) void sym_kernel(SymArgs a) {
     for (int l = 0; l < ST / 256; ++l) {
       float v[R::SRC];
       R::load(a, gb * ST + l * 256 + tid, v);
+      const int o = lds_slot<U>(l * 256 + tid);
 #pragma unroll
-      for (int e = 0; e < R::SRC; ++e) src[e * ST + l * 256 + tid] = v[e];
+      for (int e = 0; e < R::SRC; ++e) src[e * ST + o] = v[e];
     }
     __syncthreads();
 

@@ -91,6 +91,7 @@ def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4
              ("fp32_targets", nb.NB_ACC_PLAIN, targets),
              ("fp32_kahan", nb.NB_ACC_KAHAN, -1))
     for name, acc, cfg in modes:
+        print(f"precision: {name} ...", file=sys.stderr, flush=True)
         V = torch.zeros_like(P0)
         N = torch.empty_like(P0)
         nb.forward_step(unit, list(P0), list(V), list(N), acc=acc, cfg=cfg)
@@ -112,6 +113,7 @@ def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4
         damp = float(np.float32(params.damping)) ** steps
         pscale = float(np.abs(np.stack(vel).astype(np.float64)).sum())
         dp = np.array(e1["momentum"]) - damp * np.array(e0["momentum"])
+        print(f"precision: {name} {ms:.3f} ms/step", file=sys.stderr, flush=True)
         out[name] = {
             "launch": launch,
             "force_rel_err_vs_fp64": _percentiles(rel),
