@@ -146,6 +146,8 @@ const FwdConfig kConfigs[] = {
     {256, 8, 2048, 1, SCHEME_SYMMETRIC}, {256, 8, 2048, 2, SCHEME_SYMMETRIC},
     {256, 8, 2048, 4, SCHEME_SYMMETRIC}, {256, 4, 1024, 1, SCHEME_SYMMETRIC},
     {256, 4, 1024, 2, SCHEME_SYMMETRIC}, {256, 4, 1024, 4, SCHEME_SYMMETRIC},
+    {256, 1, 1024, 16}, {256, 2, 1024, 16}, {128, 1, 512, 8},
+    {128, 1, 512, 16},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 

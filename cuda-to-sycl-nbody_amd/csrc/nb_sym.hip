@@ -11,10 +11,12 @@
 //                derives it): A_k = sum_j K (h + t3 r),  h = g_j - g_k,
 //                t3 = -3 (r . h) / d, optionally fused with the forward F.
 // Both pair terms are odd under exchanging the two particles (r -> -r,
-// h -> -h, d, K, t3 unchanged), and fma(-u, K, S) is the exact negation of
-// fma(u, K, A), so the term added to particle j is bit for bit the negative
-// of the one added to particle i.  Only the order in which each particle's
-// terms are summed differs from the target-sharded kernels.
+// h -> -h, d, K, t3 unchanged).  The reaction on particle j is accumulated
+// negated, S' = fma(u, K, S'), which is bit for bit -fma(-u, K, S) (rounding
+// is symmetric under negation), and subtracted in the finalize: particle j
+// receives exactly the negative of the term added to particle i.  Only the
+// order in which each particle's terms are summed differs from the
+// target-sharded kernels.
 //
 // Decomposition (U particles per lane, P = 64 U per wave, ST = 4 P):
 //   * particles are grouped in super-tiles of ST; a work item is one
@@ -24,8 +26,8 @@
 //     VGPRs; the source super-tile sits in LDS;
 //   * for GB > GA, in pass b wave w meets source tile j = (w + b) & 3; in
 //     each of 64 steps lane l takes the U sources of set (l + s) & 63 from
-//     LDS and evaluates the U x U pair terms, adding the reaction to U
-//     per-source sums that rotate one lane down the wave (DPP wave_rol:1)
+//     LDS and evaluates the U x U pair terms, adding the (negated) reaction
+//     to U per-source sums that rotate one lane down the wave (DPP wave_rol:1)
 //     after each step, so that after 64 steps lane l again holds set l's
 //     sums.  Between passes the reaction sums of the four tiles live in LDS
 //     (each tile is owned by one wave per pass); after pass 3 the chunk
@@ -34,8 +36,8 @@
 //     all four tiles on its targets without reactions (self terms are +0);
 //   * the targets' sums over the chunk are one row partial;
 //   * a finalize kernel sums, per particle, its row partials in chunk order
-//     and then the reaction partials of its column in GA order, and applies
-//     the rule's epilogue.  The order is fixed: results are deterministic.
+//     and then subtracts the reaction partials of its column in GA order,
+//     and applies the rule's epilogue.  The order is fixed: results are deterministic.
 // Padding (index >= n): ForceRule puts padding particles at 1e30 in every
 // coordinate, so any pair with a real particle has |r|^2 = +inf,
 // rsqrt(inf) = 0 and r * 0 = 0 (exact zeros, no mask).  AdjointRule masks
@@ -111,7 +113,9 @@ struct ForceRule {
     v[2] = ok ? a.pz[j] : kFar;
   }
 
-  // Source s on target t: a += r K; with SYM the reaction -r K goes to rs.
+  // Source s on target t: a += r K.  With SYM, rs accumulates +r K too: rs
+  // holds the negated reaction sum of the source (the finalize subtracts it),
+  // which keeps every accumulation a two-operand v_fmac_f32.
   template <bool SYM, bool MASK>
   static __device__ __forceinline__ void pair(const float (&s)[SRC],
                                               const float (&t)[SRC],
@@ -129,9 +133,9 @@ struct ForceRule {
     a[1] = __builtin_fmaf(ry, K, a[1]);
     a[2] = __builtin_fmaf(rz, K, a[2]);
     if constexpr (SYM) {
-      rs[0] = __builtin_fmaf(-rx, K, rs[0]);
-      rs[1] = __builtin_fmaf(-ry, K, rs[1]);
-      rs[2] = __builtin_fmaf(-rz, K, rs[2]);
+      rs[0] = __builtin_fmaf(rx, K, rs[0]);
+      rs[1] = __builtin_fmaf(ry, K, rs[1]);
+      rs[2] = __builtin_fmaf(rz, K, rs[2]);
     }
   }
 
@@ -196,19 +200,19 @@ struct AdjointRule {
     a[0] = __builtin_fmaf(K, ux, a[0]);
     a[1] = __builtin_fmaf(K, uy, a[1]);
     a[2] = __builtin_fmaf(K, uz, a[2]);
-    if constexpr (SYM) {
-      rs[0] = __builtin_fmaf(-K, ux, rs[0]);
-      rs[1] = __builtin_fmaf(-K, uy, rs[1]);
-      rs[2] = __builtin_fmaf(-K, uz, rs[2]);
+    if constexpr (SYM) { // negated reaction sums, as in ForceRule
+      rs[0] = __builtin_fmaf(K, ux, rs[0]);
+      rs[1] = __builtin_fmaf(K, uy, rs[1]);
+      rs[2] = __builtin_fmaf(K, uz, rs[2]);
     }
     if constexpr (FUSED) {
       a[3] = __builtin_fmaf(rx, K, a[3]);
       a[4] = __builtin_fmaf(ry, K, a[4]);
       a[5] = __builtin_fmaf(rz, K, a[5]);
       if constexpr (SYM) {
-        rs[3] = __builtin_fmaf(-rx, K, rs[3]);
-        rs[4] = __builtin_fmaf(-ry, K, rs[4]);
-        rs[5] = __builtin_fmaf(-rz, K, rs[5]);
+        rs[3] = __builtin_fmaf(rx, K, rs[3]);
+        rs[4] = __builtin_fmaf(ry, K, rs[4]);
+        rs[5] = __builtin_fmaf(rz, K, rs[5]);
       }
     }
   }
@@ -402,8 +406,8 @@ __global__ __launch_bounds__(256) void sym_kernel(SymArgs a) {
     for (int e = 0; e < R::ACC; ++e) fp[e * ST + k] = T.a[k][e];
 }
 
-// Per particle: row partials (chunk order), then the reaction partials of
-// its column (ga order), then the rule's epilogue.
+// Per particle: row partials (chunk order), minus the negated reaction
+// partials of its column (ga order), then the rule's epilogue.
 template <class R, int U>
 __global__ __launch_bounds__(256) void sym_finalize(SymArgs a) {
   constexpr int ST = kWaves * 64 * U;
@@ -421,10 +425,10 @@ __global__ __launch_bounds__(256) void sym_finalize(SymArgs a) {
 #pragma unroll
     for (int e = 0; e < R::ACC; ++e) f[e] = f[e] + p[e * ST];
   }
-  for (int ga = 0; ga < g; ++ga) {
+  for (int ga = 0; ga < g; ++ga) { // negated reaction sums
     const float *p = a.spart + (size_t)pair_slot(ga, g, ts) * (R::ACC * ST) + o;
 #pragma unroll
-    for (int e = 0; e < R::ACC; ++e) f[e] = f[e] + p[e * ST];
+    for (int e = 0; e < R::ACC; ++e) f[e] = f[e] - p[e * ST];
   }
   R::finish(a, i, f);
 }

@@ -39,12 +39,13 @@ def _torch():
     return torch
 
 
-def mixed_ics(n: int, seed: int = 7, G: float = 2.0):
-    """Half Plummer sphere (a = 1, virial for G * n/2), half reference disk."""
+def mixed_ics(n: int, seed: int = 7, G: float = 2.0, a: float = 1.0):
+    """Half Plummer sphere (scale radius a, virial for G * n/2), half
+    reference disk."""
     from . import disk_ic, plummer_ic
 
     half = n // 2
-    (px, py, pz), (vx, vy, vz) = plummer_ic(half, seed=seed, a=1.0, g_mass=G * half)
+    (px, py, pz), (vx, vy, vz) = plummer_ic(half, seed=seed, a=a, g_mass=G * half)
     (dx, dy, dz), (ux, uy, uz) = disk_ic(n - half, seed=seed)
     pz = pz + np.float32(2.0)  # the disk spans z in [0, 4)
     cat = lambda a, b: np.ascontiguousarray(np.concatenate([a, b]), np.float32)
@@ -56,13 +57,16 @@ def _percentiles(e: np.ndarray) -> dict:
         "max": float(e.max())}
 
 
-def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4096) -> dict:
+def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4096,
+              plummer_a: float = 1.0, damping: float | None = None) -> dict:
     import nbody_amd as nb
 
     torch = _torch()
     dev = torch.device("cuda:0")
-    pos, vel = mixed_ics(n, seed)
+    pos, vel = mixed_ics(n, seed, a=plummer_a)
     params = nb.default_params(num_particles=n)
+    if damping is not None:
+        params.damping = damping
     P0 = torch.from_numpy(np.stack(pos)).to(dev)
     V0 = torch.from_numpy(np.stack(vel)).to(dev)
     rng = np.random.default_rng(seed)
@@ -79,7 +83,9 @@ def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4
     # fp32 force of a mode: one step with G = dt = 1, damping = 0, v = 0 gives v' = F
     unit = nb.default_params(num_particles=n, G=1.0, dt=1.0, damping=0.0)
     out = {"n": n, "steps": steps, "seed": seed, "sample": sample,
-           "ics": "half Plummer (a=1, virial for G*N/2, centred at z=2) + half reference disk",
+           "damping": float(params.damping),
+           "ics": f"half Plummer (a={plummer_a:g}, virial for G*N/2, centred at z=2) "
+                  "+ half reference disk",
            "fp64_oracle": {"seconds": t_f64,
                            "value": sample * n / t_f64 / 1e9, "unit": "Ginteractions/s"}}
     e0 = nb.energy_momentum(params, P0, V0)
@@ -99,8 +105,8 @@ def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4
         F = V[:, idx.long()].double().cpu().numpy()
         rel = np.linalg.norm(F - f64, axis=0) / norm64
         # trajectory: `steps` steps from the same state, timed
-        sim = nb.Simulator(nb.default_params(num_particles=n, sim_iterations_per_frame=steps),
-                           pos, vel)
+        sim = nb.Simulator(nb.default_params(num_particles=n, sim_iterations_per_frame=steps,
+                                             damping=params.damping), pos, vel)
         sim.set_accumulation(acc)
         sim.set_config(cfg)
         launch = nb.forward_config_info(sim.config)
@@ -128,22 +134,34 @@ def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4
 
 
 def adjoint(n: int = 16384, steps: int = 200, every: int = 20, fd_coords: int = 32,
-            h: float = 1e-3, arrays: bool = False):
+            h: float = 1e-3, arrays: bool = False, eps: float | None = None,
+            target: str = "perturbed"):
     """Returns the result dict; with arrays=True also a dict of the inputs,
-    target and gradients (numpy) for an external check."""
+    target and gradients (numpy) for an external check.
+
+    target "perturbed": the final positions of a run from initial positions
+    plus Gaussian noise (sigma 1e-3); "far": the initial positions shifted by
+    10 units in every coordinate (a residual far above fp32 rounding, for a
+    well-conditioned finite-difference check).  eps: dist_eps (None: the
+    SimParam default)."""
     import nbody_amd as nb
 
     torch = _torch()
     dev = torch.device("cuda:0")
     pos, vel = nb.disk_ic(n)
     params = nb.default_params(num_particles=n)
+    if eps is not None:
+        params.dist_eps = eps
     x0 = torch.from_numpy(np.stack(pos)).to(dev)
     v0 = torch.from_numpy(np.stack(vel)).to(dev)
     rng = np.random.default_rng(0)
-    pert = torch.from_numpy(rng.normal(scale=1e-3, size=(3, n)).astype(np.float32)).to(dev)
-    with torch.no_grad():
-        target, _ = nb.trajectory(x0 + pert, v0, params, steps, every)
-    target64 = target.double()
+    if target == "far":
+        tgt = (x0 + 10.0).contiguous()
+    else:
+        pert = torch.from_numpy(rng.normal(scale=1e-3, size=(3, n)).astype(np.float32)).to(dev)
+        with torch.no_grad():
+            tgt, _ = nb.trajectory(x0 + pert, v0, params, steps, every)
+    target64 = tgt.double()
 
     def loss_of(x, v):
         xT, _ = nb.trajectory(x, v, params, steps, every)
@@ -161,7 +179,8 @@ def adjoint(n: int = 16384, steps: int = 200, every: int = 20, fd_coords: int = 
     t2 = time.perf_counter()
     gx = x.grad.double().cpu().numpy()
     gv = v.grad.double().cpu().numpy()
-    out = {"n": n, "steps": steps, "every": every, "loss": float(L.item()),
+    out = {"n": n, "steps": steps, "every": every, "dist_eps": float(params.dist_eps),
+           "target": target, "loss": float(L.item()),
            "forward_seconds": t1 - t0, "backward_seconds": t2 - t1}
 
     # central differences with the fp32 GPU forward (fp64 loss)
@@ -198,12 +217,19 @@ def main(argv=None) -> int:
     ap.add_argument("--n", type=int)
     ap.add_argument("--steps", type=int)
     ap.add_argument("--every", type=int, default=20)
+    ap.add_argument("--eps", type=float, help="adjoint: dist_eps (default: SimParam's)")
+    ap.add_argument("--target", choices=["perturbed", "far"], default="perturbed")
+    ap.add_argument("--plummer-a", type=float, default=1.0,
+                    help="precision: Plummer scale radius")
+    ap.add_argument("--damping", type=float, help="precision: damping (default: SimParam's)")
     a = ap.parse_args(argv)
     _lib.load()
     if a.study == "precision":
-        res = precision(n=a.n or 1048576, steps=a.steps or 100)
+        res = precision(n=a.n or 1048576, steps=a.steps or 100, plummer_a=a.plummer_a,
+                        damping=a.damping)
     else:
-        res = adjoint(n=a.n or 16384, steps=a.steps or 200, every=a.every)
+        res = adjoint(n=a.n or 16384, steps=a.steps or 200, every=a.every, eps=a.eps,
+                      target=a.target)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -31,6 +31,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--every", type=int, default=20)
+    ap.add_argument("--eps", type=float, help="dist_eps (default: SimParam's)")
+    ap.add_argument("--target", choices=["perturbed", "far"], default="perturbed")
     a = ap.parse_args()
     from nbody_amd import studies
 
@@ -40,8 +42,12 @@ def main() -> int:
 
     import nbody_amd as nb
 
-    res, arr = studies.adjoint(n=a.n, steps=a.steps, every=a.every, arrays=True)
+    res, arr = studies.adjoint(n=a.n, steps=a.steps, every=a.every, arrays=True, eps=a.eps,
+                               target=a.target)
     params = nb.default_params(num_particles=a.n)
+    if a.eps is not None:
+        params.dist_eps = a.eps
+    p = orc.params() if a.eps is None else orc.params(dist_eps=a.eps)
     P = torch.from_numpy(arr["x0"]).cuda()
     V = torch.from_numpy(arr["v0"]).cuda()
     states = [arr["x0"]]
@@ -51,7 +57,6 @@ def main() -> int:
         P = N
         states.append(P.cpu().numpy())
     t0 = time.perf_counter()
-    p = orc.params()
     lx = 2.0 * (states[-1].astype(np.float64) - arr["target"]) / a.n
     lv = np.zeros_like(lx)
     for t in range(a.steps - 1, -1, -1):
@@ -62,8 +67,7 @@ def main() -> int:
                                "grad_x_rel_err": rel(arr["gx"], lx),
                                "grad_v_rel_err": rel(arr["gv"], lv)}
     t0 = time.perf_counter()
-    L64, gx64, gv64 = orc.traj_grad_f64(orc.params(), arr["x0"], arr["v0"], arr["target"],
-                                        a.steps)
+    L64, gx64, gv64 = orc.traj_grad_f64(p, arr["x0"], arr["v0"], arr["target"], a.steps)
     g64 = np.array([(gx64 if w == 0 else gv64)[c, i] for (w, c, i) in arr["coords"]])
     res["fp64_independent_trajectory"] = {
         "seconds": time.perf_counter() - t0,
