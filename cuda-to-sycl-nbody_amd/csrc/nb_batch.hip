@@ -17,8 +17,11 @@
 // memset between them.  Two steps are captured per graph (A -> B, B -> A).
 //
 // The pair loop and integrator are those of the single-system kernel
-// (nb_pairs.hpp).  A project-written autotuner times every compiled
-// (block, targets-per-thread, tile) shape on the batch itself and keeps the
+// (nb_pairs.hpp).  The symmetric shapes run the batched form of nb_sym.hip
+// instead: work item = one super-tile pair of one system, each pair term
+// evaluated once for both particles, then a finalize launch (results equal
+// the single-system symmetric step bit for bit).  A project-written
+// autotuner times every compiled shape on the batch itself and keeps the
 // fastest.
 
 #include <algorithm>
@@ -111,7 +114,8 @@ BatchFn batch_shape(int mode) {
 
 struct BatchShape {
   int block, unroll, tile;
-  BatchFn (*get)(int);
+  BatchFn (*get)(int);            // target-sharded kernels (nullptr: symmetric)
+  int scheme = SCHEME_TARGETS;
 };
 
 const BatchShape kBatchShapes[] = {
@@ -123,6 +127,9 @@ const BatchShape kBatchShapes[] = {
     {512, 1, 2048, batch_shape<512, 1, 2048>},
     {64, 1, 256, batch_shape<64, 1, 256>},
     {64, 2, 256, batch_shape<64, 2, 256>},
+    // symmetric (nb_sym.hip): 4 waves, particles per lane, super-tile
+    {256, 4, 1024, nullptr, SCHEME_SYMMETRIC},
+    {256, 8, 2048, nullptr, SCHEME_SYMMETRIC},
 };
 constexpr int kNumBatchShapes = sizeof(kBatchShapes) / sizeof(kBatchShapes[0]);
 
@@ -146,6 +153,10 @@ struct nb_batch {
   int *d_off = nullptr;
   float4 *d_par = nullptr;
   int2 *d_items = nullptr;
+  int4 *d_sym_items = nullptr; // symmetric shapes: (system, ga, gb, slot)
+  int *d_sym_slot = nullptr;   // first partial slot of each system
+  float *sym_part = nullptr;   // symmetric partial sums
+  size_t sym_part_bytes = 0;
   int *d_counters = nullptr;
   float *scratch = nullptr; // per-system split launches
   int cfg = -1, num_items = 0, grid = 0;
@@ -169,7 +180,8 @@ void destroy_batch(nb_batch *b) {
   drop_graphs(b);
   for (void *p : {(void *)b->mem, (void *)b->d_off, (void *)b->d_par,
                   (void *)b->d_items, (void *)b->d_counters,
-                  (void *)b->scratch})
+                  (void *)b->scratch, (void *)b->d_sym_items,
+                  (void *)b->d_sym_slot, (void *)b->sym_part})
     if (p) (void)hipFree(p);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -177,9 +189,61 @@ void destroy_batch(nb_batch *b) {
   delete b;
 }
 
+template <class T>
+int upload(T **dst, const std::vector<T> &host, const char *what) {
+  if (*dst) {
+    (void)hipFree(*dst);
+    *dst = nullptr;
+  }
+  if (host.empty()) return NB_OK;
+  int st = hip_check(hipMalloc(dst, sizeof(T) * host.size()), what);
+  if (st) return st;
+  return hip_check(hipMemcpy(*dst, host.data(), sizeof(T) * host.size(),
+                             hipMemcpyHostToDevice),
+                   what);
+}
+
+// Symmetric shape: work items of every system's super-tile pairs, slot
+// table, partial buffer and the persistent grid.
+int configure_sym(nb_batch *b, int cfg) {
+  const BatchShape &sh = kBatchShapes[cfg];
+  if (b->mode != MODE_PLAIN)
+    return set_error(NB_ERR_UNSUPPORTED,
+                     "symmetric batch shapes need every system's dist_eps^3 "
+                     "a normal float");
+  std::vector<int4> items;
+  std::vector<int> slots;
+  size_t bytes = 0;
+  int st = sym_batch_layout(sh.unroll, b->off.data(), b->S, items, slots, bytes);
+  if (st) return st;
+  if ((st = hip_check(hipStreamSynchronize(b->stream), "stream synchronize")) ||
+      (st = upload(&b->d_sym_items, items, "symmetric item upload")) ||
+      (st = upload(&b->d_sym_slot, slots, "symmetric slot upload")))
+    return st;
+  if (bytes > b->sym_part_bytes) {
+    if (b->sym_part) (void)hipFree(b->sym_part);
+    b->sym_part = nullptr;
+    b->sym_part_bytes = 0;
+    if ((st = hip_check(hipMalloc(&b->sym_part, bytes), "symmetric partials")))
+      return st;
+    b->sym_part_bytes = bytes;
+  }
+  b->num_items = (int)items.size();
+  int per_cu = 0;
+  if ((st = hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &per_cu, sym_batch_kernel_fn(sh.unroll), 256, 0),
+                      "occupancy query")))
+    return st;
+  b->grid = std::max(1, std::min(b->num_items, std::max(1, per_cu) * b->num_cu));
+  b->cfg = cfg;
+  drop_graphs(b);
+  return NB_OK;
+}
+
 // Builds the longest-first item list and the grid for shape `cfg`.
 int configure(nb_batch *b, int cfg) {
   const BatchShape &sh = kBatchShapes[cfg];
+  if (sh.scheme == SCHEME_SYMMETRIC) return configure_sym(b, cfg);
   const int per = sh.block * sh.unroll;
   struct Item {
     int sys, first;
@@ -198,20 +262,8 @@ int configure(nb_batch *b, int cfg) {
     host[k] = make_int2(items[k].sys, items[k].first);
   int st = hip_check(hipStreamSynchronize(b->stream), "stream synchronize");
   if (st) return st;
-  if (b->d_items) {
-    (void)hipFree(b->d_items);
-    b->d_items = nullptr;
-  }
+  if ((st = upload(&b->d_items, host, "item upload"))) return st;
   b->num_items = (int)host.size();
-  if (!host.empty()) {
-    if ((st = hip_check(hipMalloc(&b->d_items, sizeof(int2) * host.size()),
-                        "item hipMalloc")) ||
-        (st = hip_check(hipMemcpy(b->d_items, host.data(),
-                                  sizeof(int2) * host.size(),
-                                  hipMemcpyHostToDevice),
-                        "item copy")))
-      return st;
-  }
   BatchFn fn = sh.get(b->mode);
   int per_cu = 0;
   if ((st = hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -228,6 +280,25 @@ int configure(nb_batch *b, int cfg) {
 int launch_batch(nb_batch *b, int from) {
   if (b->num_items == 0) return NB_OK;
   const BatchShape &sh = kBatchShapes[b->cfg];
+  if (sh.scheme == SCHEME_SYMMETRIC) {
+    SymBatchLaunch L;
+    for (int c = 0; c < 3; ++c) {
+      L.pos[c] = b->P[from][c];
+      L.vel[c] = b->V[c];
+      L.nxt[c] = b->P[from ^ 1][c];
+    }
+    L.items = b->d_sym_items;
+    L.sys_off = b->d_off;
+    L.sys_slot = b->d_sym_slot;
+    L.sys_par = b->d_par;
+    L.part = b->sym_part;
+    L.num_items = b->num_items;
+    L.num_systems = b->S;
+    L.total = (int)b->total;
+    L.counters = b->d_counters;
+    L.grid = b->grid;
+    return launch_sym_batch(sh.unroll, L, b->stream);
+  }
   BatchArgs a;
   a.px = b->P[from][0];
   a.py = b->P[from][1];
@@ -361,6 +432,14 @@ int nb_batch_config_info(int cfg, int *block, int *unroll, int *tile) {
   if (unroll) *unroll = kBatchShapes[cfg].unroll;
   if (tile) *tile = kBatchShapes[cfg].tile;
   return NB_OK;
+}
+
+int nb_batch_config_scheme(int cfg) {
+  if (cfg < 0 || cfg >= kNumBatchShapes) {
+    set_error(NB_ERR_INVALID_ARG, "batch config out of range");
+    return -1;
+  }
+  return kBatchShapes[cfg].scheme;
 }
 
 int nb_batch_create(int num_systems, const nb_params *params, const float *px,
@@ -511,6 +590,8 @@ int nb_batch_autotune(nb_batch *b, int trial_steps, int *best_cfg) {
   int best = -1;
   float best_ms = 0.0f;
   for (int cfg = 0; cfg < kNumBatchShapes; ++cfg) {
+    if (kBatchShapes[cfg].scheme == SCHEME_SYMMETRIC && b->mode != MODE_PLAIN)
+      continue; // needs a vanishing self term in every system
     if ((st = configure(b, cfg)) || (st = timed(b, 1, false, false)))
       return st; // warm-up step
     if ((st = copy_state(b, false))) return st;

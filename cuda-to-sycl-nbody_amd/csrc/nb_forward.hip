@@ -178,8 +178,14 @@ int forward_default_config(size_t n, int mode) {
     if (n >= ((size_t)1 << 17)) return 22; // symmetric, 8 per lane, 1 per item
     if (n >= 40960) return 25;             // symmetric, 4 per lane, 1 per item
   }
-  if (n >= (size_t)1 << 17) return 11; // 256 x 4, tile 1024, split 8
-  return 3;                            // 256 x 1, tile 1024, split 8
+  // Target-sharded, Gint/s: n = 4096: 128 x 1 split 8 629 (256 x 1 split 8
+  // 480); 8192: 256 x 1 split 8 1746 (split 16 1629); 16384 .. 32768:
+  // 256 x 1 split 16 2843 .. 3432 (split 8 2792 .. 3292); 65536: 256 x 4
+  // split 8 3402 (256 x 1 split 8 3279).
+  if (n >= (size_t)1 << 16) return 11; // 256 x 4, tile 1024, split 8
+  if (n >= 12288) return 28;           // 256 x 1, tile 1024, split 16
+  if (n >= 6144) return 3;             // 256 x 1, tile 1024, split 8
+  return 30;                           // 128 x 1, tile 512, split 8
 }
 
 size_t forward_scratch_bytes(int cfg, size_t n) {

@@ -10,6 +10,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 namespace nb {
 
@@ -76,6 +77,29 @@ int launch_adjoint_sym(const StepConsts &c, bool fused, int unroll, int chunk,
                        float *const lv[3], float *const vel[3],
                        float *const nxt[3], int n, float *scratch,
                        hipStream_t stream);
+// Batched symmetric step (nb_sym.hip), plain mode only.  Layout of the work
+// items (system, ga, gb, slot), longest first, the first slot of each system
+// ([S + 1]) and the bytes of the partial buffer, for systems given by their
+// offsets (sys_off[S + 1]).
+int sym_batch_layout(int unroll, const int *sys_off, int num_systems,
+                     std::vector<int4> &items, std::vector<int> &sys_slot,
+                     size_t &part_bytes);
+struct SymBatchLaunch {
+  const float *pos[3];
+  float *vel[3];
+  float *nxt[3];
+  const int4 *items;
+  const int *sys_off, *sys_slot;
+  const float4 *sys_par;
+  float *part;
+  int num_items, num_systems, total;
+  int *counters; // two self-resetting queue counters (zero before first use)
+  int grid;      // persistent workgroups
+};
+// Pair kernel of the batched symmetric step (for occupancy queries).
+const void *sym_batch_kernel_fn(int unroll);
+int launch_sym_batch(int unroll, const SymBatchLaunch &L, hipStream_t stream);
+
 int launch_forward_sym(const StepConsts &c, int unroll, int chunk,
                        const float *px, const float *py, const float *pz,
                        float *vx, float *vy, float *vz, float *nx, float *ny,

@@ -54,6 +54,8 @@
 
 #include "nb_pairs.hpp"
 
+#include <vector>
+
 namespace nb {
 namespace {
 
@@ -333,6 +335,77 @@ __device__ __forceinline__ void super_pair(Lane<R, U> &L, const float *src,
   }
 }
 
+// Targets of super-tile ga held by this lane (wave w): inputs, zeroed sums
+// and validity (index < n).
+template <class R, int U>
+__device__ __forceinline__ void load_targets(Lane<R, U> &T, bool (&vt)[U],
+                                             const SymArgs &a, int ga, int w,
+                                             int lane) {
+  constexpr int P = 64 * U;
+  constexpr int ST = kWaves * P;
+  const int t0 = ga * ST + w * P + lane * U;
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    R::load(a, t0 + k, T.t[k]);
+    vt[k] = t0 + k < a.n;
+#pragma unroll
+    for (int e = 0; e < R::ACC; ++e) T.a[k][e] = 0.0f;
+  }
+}
+
+// Super-tile pair (ga, gb), gb >= ga: stages source super-tile gb in LDS,
+// adds its terms to the targets' sums and, for gb > ga, writes the negated
+// reaction sums of its particles to sp ([ACC][ST] floats).  Called by every
+// thread of the workgroup (it contains barriers).
+template <class R, int U>
+__device__ __forceinline__ void pair_step(Lane<R, U> &T, const bool (&vt)[U],
+                                          const SymArgs &a, float *src,
+                                          float *sacc, int ga, int gb,
+                                          float *sp) {
+  constexpr int P = 64 * U;
+  constexpr int ST = kWaves * P;
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, lane = tid & 63;
+  const int n = a.n;
+  __syncthreads(); // the previous super-tile and its sums are written out
+#pragma unroll
+  for (int l = 0; l < ST / 256; ++l) {
+    float v[R::SRC];
+    R::load(a, gb * ST + l * 256 + tid, v);
+    const int o = lds_slot<U>(l * 256 + tid);
+#pragma unroll
+    for (int e = 0; e < R::SRC; ++e) src[e * ST + o] = v[e];
+  }
+  __syncthreads();
+
+  // only the last super-tile holds padding particles (gb >= ga)
+  bool masked = false;
+  if constexpr (R::MASKED_TAIL) masked = (n % ST) != 0 && (gb + 1) * ST > n;
+  if (masked)
+    super_pair<R, U, R::MASKED_TAIL>(T, src, sacc, gb == ga, w, lane, a.c.eps,
+                                     vt, gb * ST, n);
+  else
+    super_pair<R, U, false>(T, src, sacc, gb == ga, w, lane, a.c.eps, vt,
+                            gb * ST, n);
+  if (gb == ga) return;
+#pragma unroll
+  for (int l = 0; l < R::ACC * ST / 256; ++l)
+    sp[l * 256 + tid] = sacc[l * 256 + tid];
+}
+
+// The targets' sums as one row partial ([ACC][ST] floats at fp).
+template <class R, int U>
+__device__ __forceinline__ void store_row(const Lane<R, U> &T, float *fp,
+                                          int w, int lane) {
+  constexpr int P = 64 * U;
+  constexpr int ST = kWaves * P;
+  float *q = fp + w * P + lane * U;
+#pragma unroll
+  for (int k = 0; k < U; ++k)
+#pragma unroll
+    for (int e = 0; e < R::ACC; ++e) q[e * ST + k] = T.a[k][e];
+}
+
 template <class R, int U>
 __global__ __launch_bounds__(256) void sym_kernel(SymArgs a) {
   constexpr int P = 64 * U;
@@ -341,9 +414,8 @@ __global__ __launch_bounds__(256) void sym_kernel(SymArgs a) {
   __shared__ float4 sacc4[R::ACC][ST / 4]; // reaction sums of its 4 tiles
   float *src = reinterpret_cast<float *>(src4);
   float *sacc = reinterpret_cast<float *>(sacc4);
-  const int tid = threadIdx.x;
-  const int w = tid >> 6, lane = tid & 63;
-  const int n = a.n, ts = a.ts, L = a.chunk;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ts = a.ts, L = a.chunk;
 
   // Work item -> (row ga, chunk c): rows are enumerated in order, row ga
   // holding ceil((ts - ga) / L) items.
@@ -358,52 +430,15 @@ __global__ __launch_bounds__(256) void sym_kernel(SymArgs a) {
   const int c = (int)(it - row_first(ga, ts, L));
   const int gb0 = ga + c * L;
   const int gb1 = min(gb0 + L, ts);
-  const bool ragged = (n % ST) != 0;
 
   Lane<R, U> T;
   bool vt[U];
-  const int t0 = ga * ST + w * P + lane * U;
-#pragma unroll
-  for (int k = 0; k < U; ++k) {
-    R::load(a, t0 + k, T.t[k]);
-    vt[k] = t0 + k < n;
-#pragma unroll
-    for (int e = 0; e < R::ACC; ++e) T.a[k][e] = 0.0f;
-  }
-
-  for (int gb = gb0; gb < gb1; ++gb) {
-    __syncthreads(); // the previous super-tile and its sums are written out
-#pragma unroll
-    for (int l = 0; l < ST / 256; ++l) {
-      float v[R::SRC];
-      R::load(a, gb * ST + l * 256 + tid, v);
-      const int o = lds_slot<U>(l * 256 + tid);
-#pragma unroll
-      for (int e = 0; e < R::SRC; ++e) src[e * ST + o] = v[e];
-    }
-    __syncthreads();
-
-    // only the last super-tile holds padding particles (gb >= ga)
-    bool masked = false;
-    if constexpr (R::MASKED_TAIL) masked = ragged && gb == ts - 1;
-    if (masked)
-      super_pair<R, U, R::MASKED_TAIL>(T, src, sacc, gb == ga, w, lane,
-                                       a.c.eps, vt, gb * ST, n);
-    else
-      super_pair<R, U, false>(T, src, sacc, gb == ga, w, lane, a.c.eps, vt,
-                              gb * ST, n);
-    if (gb == ga) continue;
-    float *sp = a.spart + (size_t)pair_slot(ga, gb, ts) * (R::ACC * ST);
-#pragma unroll
-    for (int l = 0; l < R::ACC * ST / 256; ++l)
-      sp[l * 256 + tid] = sacc[l * 256 + tid];
-  }
-
-  float *fp = a.fpart + (size_t)it * (R::ACC * ST) + w * P + lane * U;
-#pragma unroll
-  for (int k = 0; k < U; ++k)
-#pragma unroll
-    for (int e = 0; e < R::ACC; ++e) fp[e * ST + k] = T.a[k][e];
+  load_targets(T, vt, a, ga, w, lane);
+  for (int gb = gb0; gb < gb1; ++gb)
+    pair_step(T, vt, a, src, sacc, ga, gb,
+              gb > ga ? a.spart + (size_t)pair_slot(ga, gb, ts) * (R::ACC * ST)
+                      : nullptr);
+  store_row(T, a.fpart + (size_t)it * (R::ACC * ST), w, lane);
 }
 
 // Per particle: row partials (chunk order), minus the negated reaction
@@ -431,6 +466,135 @@ __global__ __launch_bounds__(256) void sym_finalize(SymArgs a) {
     for (int e = 0; e < R::ACC; ++e) f[e] = f[e] - p[e * ST];
   }
   R::finish(a, i, f);
+}
+
+// ---- many independent systems (batched persistent launch) ----------------
+//
+// Work item = (system s, super-tile pair ga <= gb of s, slot); the items of
+// all systems are handed out longest first through a device counter to a
+// grid sized to the GPU's resident capacity.  Slot k owns a row partial
+// part[k][0] and, for ga < gb, a reaction partial part[k][1] (ACC x ST
+// floats each); the slots of system s are sys_slot[s] + the row-major index
+// of (ga, gb) in its upper triangle.  The finalize sums, per particle, the
+// row partials of its row in gb order minus the reaction partials of its
+// column in ga order: the same values in the same order as the
+// single-system kernel with one super-tile per item, so a batched step
+// equals the systems' single-system steps bit for bit.
+
+struct SymBatchArgs {
+  const float *px, *py, *pz; // concatenated systems
+  float *vx, *vy, *vz;
+  float *nx, *ny, *nz;
+  const int4 *items;     // (system, ga, gb, slot)
+  const int *sys_off;    // [S + 1] first particle of each system
+  const int *sys_slot;   // [S + 1] first slot of each system
+  const float4 *sys_par; // (eps, damping, dt, G)
+  float *part;           // [slots][2][3][ST]
+  int num_items;
+  int num_systems;
+  int total; // particles in all systems
+  int *counters; // [0]: next item, [1]: workgroups finished
+};
+
+// The single-system view of system s (arrays shifted to its first particle).
+__device__ __forceinline__ SymArgs system_view(const SymBatchArgs &b, int s,
+                                               int ST) {
+  const int off0 = b.sys_off[s];
+  const float4 pr = b.sys_par[s];
+  SymArgs a = {};
+  a.px = b.px + off0;
+  a.py = b.py + off0;
+  a.pz = b.pz + off0;
+  a.vx = b.vx + off0;
+  a.vy = b.vy + off0;
+  a.vz = b.vz + off0;
+  a.nx = b.nx + off0;
+  a.ny = b.ny + off0;
+  a.nz = b.nz + off0;
+  a.n = b.sys_off[s + 1] - off0;
+  a.ts = (a.n + ST - 1) / ST;
+  a.chunk = 1;
+  a.c.eps = pr.x;
+  a.c.damping = pr.y;
+  a.c.dt = pr.z;
+  a.c.G = pr.w;
+  return a;
+}
+
+__host__ __device__ inline long long tri_slot(int ga, int gb, int ts) {
+  // row-major index of (ga, gb), ga <= gb, in the upper triangle with diagonal
+  return (long long)ga * ts - (long long)ga * (ga - 1) / 2 + (gb - ga);
+}
+
+template <int U>
+__global__ __launch_bounds__(256) void sym_batch_kernel(SymBatchArgs b) {
+  using R = ForceRule;
+  constexpr int P = 64 * U;
+  constexpr int ST = kWaves * P;
+  __shared__ float4 src4[R::SRC][ST / 4];
+  __shared__ float4 sacc4[R::ACC][ST / 4];
+  __shared__ int s_item;
+  float *src = reinterpret_cast<float *>(src4);
+  float *sacc = reinterpret_cast<float *>(sacc4);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(b.counters, 1);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads(); // every lane has read s_item before it is rewritten
+    if (item >= b.num_items) break;
+    const int4 it = b.items[item];
+    const SymArgs a = system_view(b, it.x, ST);
+    float *slot = b.part + (size_t)it.w * (2 * R::ACC * ST);
+    Lane<R, U> T;
+    bool vt[U];
+    load_targets(T, vt, a, it.y, w, lane);
+    pair_step(T, vt, a, src, sacc, it.y, it.z, slot + R::ACC * ST);
+    store_row(T, slot, w, lane);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence(); // this workgroup's last queue access precedes the count
+    const int done = atomicAdd(b.counters + 1, 1);
+    if (done == (int)gridDim.x - 1) {
+      atomicExch(b.counters, 0);
+      atomicExch(b.counters + 1, 0);
+    }
+  }
+}
+
+template <int U>
+__global__ __launch_bounds__(256) void sym_batch_finalize(SymBatchArgs b) {
+  using R = ForceRule;
+  constexpr int ST = kWaves * 64 * U;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= b.total) return;
+  // system of particle i: the last s with sys_off[s] <= i (empty systems
+  // share their offset with the next one and are skipped)
+  int lo = 0, hi = b.num_systems - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b.sys_off[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const SymArgs a = system_view(b, lo, ST);
+  const int li = i - b.sys_off[lo];
+  const int g = li / ST, o = li % ST;
+  const float *base = b.part + (size_t)b.sys_slot[lo] * (2 * R::ACC * ST) + o;
+  float f[R::ACC];
+#pragma unroll
+  for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
+  for (int gb = g; gb < a.ts; ++gb) {
+    const float *p = base + (size_t)tri_slot(g, gb, a.ts) * (2 * R::ACC * ST);
+#pragma unroll
+    for (int e = 0; e < R::ACC; ++e) f[e] = f[e] + p[e * ST];
+  }
+  for (int ga = 0; ga < g; ++ga) { // negated reaction sums
+    const float *p = base + (size_t)tri_slot(ga, g, a.ts) * (2 * R::ACC * ST) +
+                     R::ACC * ST;
+#pragma unroll
+    for (int e = 0; e < R::ACC; ++e) f[e] = f[e] - p[e * ST];
+  }
+  R::finish(a, li, f);
 }
 
 using KernelFn = void (*)(SymArgs);
@@ -488,6 +652,85 @@ size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc) {
   const long long ts = ((long long)n + st - 1) / st;
   const long long slots = items_upto(ts, chunk) + ts * (ts - 1) / 2;
   return sizeof(float) * (size_t)slots * acc * (size_t)st;
+}
+
+int sym_batch_layout(int unroll, const int *sys_off, int num_systems,
+                     std::vector<int4> &items, std::vector<int> &sys_slot,
+                     size_t &part_bytes) {
+  if (unroll != 4 && unroll != 8)
+    return set_error(NB_ERR_UNSUPPORTED, "no symmetric batch kernel for shape");
+  const int ST = kWaves * 64 * unroll;
+  std::vector<int4> off_diag, diag;
+  sys_slot.assign(num_systems + 1, 0);
+  long long slots = 0;
+  for (int s = 0; s < num_systems; ++s) {
+    sys_slot[s] = (int)slots;
+    const int n = sys_off[s + 1] - sys_off[s];
+    const int ts = (n + ST - 1) / ST;
+    for (int ga = 0; ga < ts; ++ga)
+      for (int gb = ga; gb < ts; ++gb) {
+        const int4 it = make_int4(s, ga, gb, (int)(slots + tri_slot(ga, gb, ts)));
+        (gb == ga ? diag : off_diag).push_back(it);
+      }
+    slots += (long long)ts * (ts + 1) / 2;
+    if (slots > (1LL << 30))
+      return set_error(NB_ERR_UNSUPPORTED, "too many symmetric work items");
+  }
+  sys_slot[num_systems] = (int)slots;
+  // longest first: pairs with reactions, then the diagonal super-tiles
+  items = off_diag;
+  items.insert(items.end(), diag.begin(), diag.end());
+  part_bytes = sizeof(float) * (size_t)slots * 2 * 3 * ST;
+  return NB_OK;
+}
+
+namespace {
+template <int U> const void *batch_pairs_fn() {
+  return reinterpret_cast<const void *>(sym_batch_kernel<U>);
+}
+template <int U> const void *batch_finalize_fn() {
+  return reinterpret_cast<const void *>(sym_batch_finalize<U>);
+}
+} // namespace
+
+const void *sym_batch_kernel_fn(int unroll) {
+  return unroll == 4 ? batch_pairs_fn<4>() : unroll == 8 ? batch_pairs_fn<8>() : nullptr;
+}
+
+int launch_sym_batch(int unroll, const SymBatchLaunch &L, hipStream_t stream) {
+  const void *pairs = sym_batch_kernel_fn(unroll);
+  const void *fin = unroll == 4 ? batch_finalize_fn<4>()
+                                : unroll == 8 ? batch_finalize_fn<8>() : nullptr;
+  if (!pairs || !fin)
+    return set_error(NB_ERR_UNSUPPORTED, "no symmetric batch kernel for shape");
+  if (L.num_items == 0) return NB_OK;
+  SymBatchArgs b;
+  b.px = L.pos[0];
+  b.py = L.pos[1];
+  b.pz = L.pos[2];
+  b.vx = L.vel[0];
+  b.vy = L.vel[1];
+  b.vz = L.vel[2];
+  b.nx = L.nxt[0];
+  b.ny = L.nxt[1];
+  b.nz = L.nxt[2];
+  b.items = L.items;
+  b.sys_off = L.sys_off;
+  b.sys_slot = L.sys_slot;
+  b.sys_par = L.sys_par;
+  b.part = L.part;
+  b.num_items = L.num_items;
+  b.num_systems = L.num_systems;
+  b.total = L.total;
+  b.counters = L.counters;
+  void *args[] = {&b};
+  int r = hip_check(hipLaunchKernel(pairs, dim3(L.grid), dim3(256), args, 0,
+                                    stream),
+                    "symmetric batch launch");
+  if (r) return r;
+  return hip_check(hipLaunchKernel(fin, dim3((L.total + 255) / 256),
+                                   dim3(256), args, 0, stream),
+                   "symmetric batch finalize launch");
 }
 
 int launch_forward_sym(const StepConsts &c, int unroll, int chunk,

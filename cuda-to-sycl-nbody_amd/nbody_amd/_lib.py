@@ -142,6 +142,7 @@ def load() -> ctypes.CDLL:
     _sig(lib, "nb_batch_config", c_int, [vp])
     _sig(lib, "nb_batch_config_count", c_int, [])
     _sig(lib, "nb_batch_config_info", c_int, [c_int] + [POINTER(c_int)] * 3)
+    _sig(lib, "nb_batch_config_scheme", c_int, [c_int])
     _sig(lib, "nb_batch_last_ms", c_float, [vp])
     _sig(lib, "nb_batch_num_particles", c_size_t, [vp])
     _sig(lib, "nb_batch_read_pos", c_int, [vp, vp, vp, vp])

@@ -12,7 +12,7 @@ from typing import Sequence
 
 import numpy as np
 
-from ._lib import NbParams, check, load
+from ._lib import NB_ERR_INVALID_ARG, NbParams, check, load
 
 
 def _p(a: np.ndarray) -> ctypes.c_void_p:
@@ -25,9 +25,17 @@ def batch_config_count() -> int:
 
 
 def batch_config_info(cfg: int) -> dict:
+    """Batch shape `cfg`: block, unroll, tile and scheme ("targets" or
+    "symmetric", where tile is the super-tile length)."""
+    lib = load()
     vals = [ctypes.c_int() for _ in range(3)]
-    check(load().nb_batch_config_info(cfg, *[ctypes.byref(v) for v in vals]))
-    return dict(zip(("block", "unroll", "tile"), (v.value for v in vals)))
+    check(lib.nb_batch_config_info(cfg, *[ctypes.byref(v) for v in vals]))
+    info = dict(zip(("block", "unroll", "tile"), (v.value for v in vals)))
+    scheme = int(lib.nb_batch_config_scheme(cfg))
+    if scheme < 0:
+        check(NB_ERR_INVALID_ARG)
+    info["scheme"] = {0: "targets", 1: "symmetric"}[scheme]
+    return info
 
 
 class Batch:

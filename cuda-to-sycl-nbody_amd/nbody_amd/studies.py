@@ -58,7 +58,8 @@ def _percentiles(e: np.ndarray) -> dict:
 
 
 def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4096,
-              plummer_a: float = 1.0, damping: float | None = None) -> dict:
+              plummer_a: float = 1.0, damping: float | None = None,
+              eps: float | None = None) -> dict:
     import nbody_amd as nb
 
     torch = _torch()
@@ -67,6 +68,8 @@ def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4
     params = nb.default_params(num_particles=n)
     if damping is not None:
         params.damping = damping
+    if eps is not None:
+        params.dist_eps = eps
     P0 = torch.from_numpy(np.stack(pos)).to(dev)
     V0 = torch.from_numpy(np.stack(vel)).to(dev)
     rng = np.random.default_rng(seed)
@@ -81,9 +84,10 @@ def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4
     norm64 = np.linalg.norm(f64, axis=0)
 
     # fp32 force of a mode: one step with G = dt = 1, damping = 0, v = 0 gives v' = F
-    unit = nb.default_params(num_particles=n, G=1.0, dt=1.0, damping=0.0)
+    unit = nb.default_params(num_particles=n, G=1.0, dt=1.0, damping=0.0,
+                             dist_eps=params.dist_eps)
     out = {"n": n, "steps": steps, "seed": seed, "sample": sample,
-           "damping": float(params.damping),
+           "damping": float(params.damping), "dist_eps": float(params.dist_eps),
            "ics": f"half Plummer (a={plummer_a:g}, virial for G*N/2, centred at z=2) "
                   "+ half reference disk",
            "fp64_oracle": {"seconds": t_f64,
@@ -106,7 +110,8 @@ def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4
         rel = np.linalg.norm(F - f64, axis=0) / norm64
         # trajectory: `steps` steps from the same state, timed
         sim = nb.Simulator(nb.default_params(num_particles=n, sim_iterations_per_frame=steps,
-                                             damping=params.damping), pos, vel)
+                                             damping=params.damping,
+                                             dist_eps=params.dist_eps), pos, vel)
         sim.set_accumulation(acc)
         sim.set_config(cfg)
         launch = nb.forward_config_info(sim.config)
@@ -217,7 +222,7 @@ def main(argv=None) -> int:
     ap.add_argument("--n", type=int)
     ap.add_argument("--steps", type=int)
     ap.add_argument("--every", type=int, default=20)
-    ap.add_argument("--eps", type=float, help="adjoint: dist_eps (default: SimParam's)")
+    ap.add_argument("--eps", type=float, help="dist_eps (default: SimParam's)")
     ap.add_argument("--target", choices=["perturbed", "far"], default="perturbed")
     ap.add_argument("--plummer-a", type=float, default=1.0,
                     help="precision: Plummer scale radius")
@@ -226,7 +231,7 @@ def main(argv=None) -> int:
     _lib.load()
     if a.study == "precision":
         res = precision(n=a.n or 1048576, steps=a.steps or 100, plummer_a=a.plummer_a,
-                        damping=a.damping)
+                        damping=a.damping, eps=a.eps)
     else:
         res = adjoint(n=a.n or 16384, steps=a.steps or 200, every=a.every, eps=a.eps,
                       target=a.target)

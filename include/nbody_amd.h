@@ -282,6 +282,11 @@ int nb_batch_set_config(nb_batch *b, int cfg);
 int nb_batch_config(const nb_batch *b);
 int nb_batch_config_count(void);
 int nb_batch_config_info(int cfg, int *block, int *unroll, int *tile);
+/* Scheme of a batch shape (NB_SCHEME_TARGETS: work item = a block of one
+ * system's targets; NB_SCHEME_SYMMETRIC: work item = a super-tile pair of one
+ * system, each pair term evaluated once, for batches whose systems all have
+ * dist_eps^3 a normal float), or -1 for a bad index. */
+int nb_batch_config_scheme(int cfg);
 float nb_batch_last_ms(const nb_batch *b);
 size_t nb_batch_num_particles(const nb_batch *b);
 int nb_batch_read_pos(nb_batch *b, float *x, float *y, float *z);

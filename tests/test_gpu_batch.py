@@ -132,3 +132,97 @@ def test_empty_and_tiny_batches(nb, gpu):
     x = b.positions()
     assert np.isfinite(np.stack(x)).all()
     b.close()
+
+
+def sym_batch_configs(nb):
+    return [c for c in range(nb.batch_config_count())
+            if nb.batch_config_info(c)["scheme"] == "symmetric"]
+
+
+def forward_twin(nb, bcfg):
+    """Single-system symmetric shape with the batch shape's super-tile and one
+    super-tile pair per work item."""
+    b = nb.batch_config_info(bcfg)
+    for c in range(nb.forward_config_count()):
+        f = nb.forward_config_info(c)
+        if (f["scheme"], f["unroll"], f["tile"], f["split"]) == ("symmetric", b["unroll"],
+                                                                  b["tile"], 1):
+            return c
+    raise AssertionError(f"no forward twin of {b}")
+
+
+SYM_SIZES = [1, 100, 1023, 1024, 1025, 3000, 0, 2048, 5000, 9000]
+
+
+def test_symmetric_batch_equals_single_system(nb, gpu):
+    """Every system of a batched symmetric step equals the single-system
+    symmetric step (same super-tile pairs, same partial order), bit for bit;
+    sizes around the super-tile length and an empty system."""
+    import torch
+
+    params, pos, vel = make_batch(nb, SYM_SIZES)
+    for bcfg in sym_batch_configs(nb):
+        fcfg = forward_twin(nb, bcfg)
+        b = nb.Batch(params, pos, vel)
+        b.set_config(bcfg)
+        b.step(1)
+        bp, bv = b.positions(), b.velocities()
+        b.close()
+        off = 0
+        for p, n in zip(params, SYM_SIZES):
+            if n:
+                dp = [torch.from_numpy(a[off:off + n].copy()).to(gpu) for a in pos]
+                dv = [torch.from_numpy(a[off:off + n].copy()).to(gpu) for a in vel]
+                dn = [torch.empty_like(t) for t in dp]
+                nb.forward_step(p, dp, dv, dn, cfg=fcfg)
+                torch.cuda.synchronize()
+                for c in range(3):
+                    assert np.array_equal(bp[c][off:off + n], dn[c].cpu().numpy()), (bcfg, n)
+                    assert np.array_equal(bv[c][off:off + n], dv[c].cpu().numpy()), (bcfg, n)
+            off += n
+
+
+def test_symmetric_batch_graph_replays(nb, gpu):
+    """Captured two-step graphs (pair kernel + finalize, twice) equal direct
+    launches on every replay, with eager reads between replays."""
+    params, pos, vel = make_batch(nb, SYM_SIZES)
+    cfg = sym_batch_configs(nb)[0]
+    g = nb.Batch(params, pos, vel)
+    d = nb.Batch(params, pos, vel)
+    g.set_config(cfg)
+    d.set_config(cfg)
+    for rep in range(4):
+        g.step(2)
+        d.step_direct(2)
+        for a, b in zip(g.positions() + g.velocities(), d.positions() + d.velocities()):
+            assert np.array_equal(a, b), rep
+    g.close()
+    d.close()
+
+
+def test_symmetric_batch_vs_oracle(nb, orc, gpu):
+    params, pos, vel = make_batch(nb, [700, 5000])
+    b = nb.Batch(params, pos, vel)
+    b.set_config(sym_batch_configs(nb)[0])
+    b.step(1)
+    bp = b.positions()
+    b.close()
+    off, n = 700, 5000
+    p = orc.params(G=params[1].G, dt=params[1].dt, num_particles=n)
+    rp, _ = orc.step_f32(p, [a[off:] for a in pos], [a[off:] for a in vel])
+    got = np.stack([a[off:] for a in bp]).astype(np.float64)
+    ref = np.stack(rp).astype(np.float64)
+    e = np.abs(got - ref).max(0) / np.maximum(np.abs(ref).max(0), 1.0)
+    print(f"symmetric batch vs oracle: max {e.max():.2e} p99 {np.percentile(e, 99):.2e}")
+    assert np.percentile(e, 99) <= 1e-5 and e.max() <= 1e-4
+
+
+def test_symmetric_batch_needs_vanishing_self_term(nb, gpu):
+    params, pos, vel = make_batch(nb, [300, 400])
+    params[1].dist_eps = 0.0
+    b = nb.Batch(params, pos, vel)
+    with pytest.raises(nb.NbError, match="symmetric"):
+        b.set_config(sym_batch_configs(nb)[0])
+    best = b.autotune(1)  # skips the symmetric shapes
+    assert nb.batch_config_info(best)["scheme"] == "targets"
+    b.close()
