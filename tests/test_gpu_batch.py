@@ -12,7 +12,8 @@ def make_batch(nb, sizes, seed0=100):
     params, pos, vel = [], [[], [], []], [[], [], []]
     for k, n in enumerate(sizes):
         p, v = nb.plummer_ic(n, seed=seed0 + k, g_mass=1.0)
-        params.append(nb.default_params(num_particles=n, G=1.0 / n, dt=0.005 * (1 + k % 3)))
+        params.append(nb.default_params(num_particles=n, G=1.0 / max(n, 1),
+                                        dt=0.005 * (1 + k % 3)))
         for c in range(3):
             pos[c].append(p[c])
             vel[c].append(v[c])
