@@ -282,6 +282,7 @@ const FwdConfig kAdjConfigs[] = {
     {256, 2, 1024, 8}, {256, 4, 1024, 8},
     {256, 4, 1024, 1, SCHEME_SYMMETRIC}, {256, 4, 1024, 2, SCHEME_SYMMETRIC},
     {256, 8, 2048, 1, SCHEME_SYMMETRIC},
+    {256, 4, 256, 1, SCHEME_SYMMETRIC_TILES},
 };
 constexpr int kNumAdj = sizeof(kAdjConfigs) / sizeof(kAdjConfigs[0]);
 
@@ -298,21 +299,26 @@ int adjoint_config_count() { return kNumAdj; }
 const FwdConfig &adjoint_config(int cfg) { return kAdjConfigs[cfg]; }
 
 int adjoint_default_config(size_t n, bool skip) {
-  // Measured on MI355X (disk ICs), adjoint / fused ms, symmetric 4 per lane
-  // vs the best target-sharded shape: n = 16384: 0.243 / 0.298 vs
-  // 0.428 / 0.471;  65536: 1.89 / 2.30 vs 2.69 / 3.09;  262144: 26.4 / 32.9
-  // vs 38.7 / 43.5.  Target-sharded: 256 x 1 split 2 at N = 262144,
-  // 256 x 2 split 4 at N = 16384..25600.
-  if (!skip && n >= ((size_t)1 << 14) && n <= kSymMaxParticles)
-    return 14; // symmetric, 4 per lane, 1 super-tile per item
-  if (n >= (size_t)1 << 17) return 8;
-  return 9;
+  // Measured on MI355X (disk ICs), adjoint / fused ms:
+  //   n = 2048: 256 x 1 split 2 0.066 / 0.071, tile pairs 0.066 / 0.079;
+  //   n = 4096 .. 65536: tile pairs 0.069 / 0.082 .. 1.72 / 2.17, super-tile
+  //     pairs 0.23 / 0.28 .. 1.84 / 2.20, best target-sharded 0.11 / 0.12 ..
+  //     2.69 / 3.09;
+  //   n = 262144: super-tile pairs 25.5 / 31.3, target-sharded 38.7 / 43.5.
+  if (!skip && n <= kSymMaxParticles) {
+    if (n >= ((size_t)1 << 17)) return 14; // symmetric super-tile pairs
+    if (n >= 3072) return 17;              // symmetric tile pairs
+  }
+  if (n >= (size_t)1 << 17) return 8; // 256 x 1, split 2
+  if (n < 4096) return 8;
+  return 9;                           // 256 x 2, split 4
 }
 
 size_t adjoint_scratch_bytes(int cfg, size_t n) {
   const FwdConfig &c = kAdjConfigs[cfg];
-  if (c.scheme == SCHEME_SYMMETRIC) // sized for the fused variant
-    return sym_scratch_bytes(c.unroll, c.split, n, 6);
+  if (c.scheme != SCHEME_TARGETS) // sized for the fused variant
+    return sym_scratch_bytes(c.unroll, c.split, n, 6,
+                             c.scheme == SCHEME_SYMMETRIC_TILES);
   return c.split > 1 ? sizeof(float) * 6 * (size_t)c.split * n : 0;
 }
 
@@ -326,12 +332,13 @@ int launch_adjoint(const StepConsts &c, bool skip, bool fused, int cfg,
     return set_error(NB_ERR_INVALID_ARG, "adjoint config out of range");
   if (n <= 0) return NB_OK;
   const FwdConfig &fc = kAdjConfigs[cfg];
-  if (fc.scheme == SCHEME_SYMMETRIC) {
+  if (fc.scheme != SCHEME_TARGETS) {
     if (skip)
       return set_error(NB_ERR_UNSUPPORTED,
                        "symmetric launch shapes need dist_eps^3 a normal "
                        "float");
-    return launch_adjoint_sym(c, fused, fc.unroll, fc.split, pos, lxn, lvn,
+    return launch_adjoint_sym(c, fused, fc.unroll, fc.split,
+                              fc.scheme == SCHEME_SYMMETRIC_TILES, pos, lxn, lvn,
                               lx, lv, vel, nxt, n, scratch, stream);
   }
   AdjFn fn = find_adj(fc, skip, fused);

@@ -148,6 +148,7 @@ const FwdConfig kConfigs[] = {
     {256, 4, 1024, 2, SCHEME_SYMMETRIC}, {256, 4, 1024, 4, SCHEME_SYMMETRIC},
     {256, 1, 1024, 16}, {256, 2, 1024, 16}, {128, 1, 512, 8},
     {128, 1, 512, 16},
+    {256, 4, 256, 1, SCHEME_SYMMETRIC_TILES},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
@@ -165,18 +166,18 @@ int forward_config_count() { return kNumConfigs; }
 const FwdConfig &forward_config(int cfg) { return kConfigs[cfg]; }
 
 int forward_default_config(size_t n, int mode) {
-  // Measured on MI355X (disk ICs), Gint/s, symmetric vs target-sharded:
-  //   n = 16384: 4 per lane 1767 vs split-8 2477;   32768: 2844 vs 2931;
-  //   n = 49152: 4 per lane 4023 vs 3227;           65536: 4214 vs 3402;
-  //   n = 2^17 .. 2^20: 8 per lane 5048 .. 5803 vs 3657 .. 3958.
+  // Measured on MI355X (disk ICs), Gint/s:
+  //   symmetric, tile pairs (4 per lane): n = 25600 3750, 32768 3847,
+  //     49152 4409, 65536 4669, 131072 4895;
+  //   symmetric, super-tile pairs: 8 per lane 5241 (2^17) .. 5967 (2^20),
+  //     4 per lane 4433 (65536);
+  //   target-sharded: 16384 2670 (split 16), 25600 2862, 65536 3636.
   // Past 2^19 two super-tiles per item run as fast and need 25 % less
-  // scratch.  Without the symmetric scheme, splitting the sources 8 ways keeps
-  // 4-8 waves per SIMD in flight, and large n also profits from 4 targets per
-  // thread.
+  // scratch.
   if (mode == MODE_PLAIN && n <= kSymMaxParticles) {
     if (n > ((size_t)1 << 19)) return 23; // symmetric, 8 per lane, 2 per item
     if (n >= ((size_t)1 << 17)) return 22; // symmetric, 8 per lane, 1 per item
-    if (n >= 40960) return 25;             // symmetric, 4 per lane, 1 per item
+    if (n >= 20480) return 32;             // symmetric tile pairs, 4 per lane
   }
   // Target-sharded, Gint/s: n = 4096: 128 x 1 split 8 629 (256 x 1 split 8
   // 480); 8192: 256 x 1 split 8 1746 (split 16 1629); 16384 .. 32768:
@@ -190,8 +191,9 @@ int forward_default_config(size_t n, int mode) {
 
 size_t forward_scratch_bytes(int cfg, size_t n) {
   const FwdConfig &c = kConfigs[cfg];
-  if (c.scheme == SCHEME_SYMMETRIC)
-    return sym_scratch_bytes(c.unroll, c.split, n, 3);
+  if (c.scheme != SCHEME_TARGETS)
+    return sym_scratch_bytes(c.unroll, c.split, n, 3,
+                             c.scheme == SCHEME_SYMMETRIC_TILES);
   return c.split > 1 ? sizeof(float) * 3 * (size_t)c.split * n : 0;
 }
 
@@ -203,12 +205,14 @@ int launch_forward(const StepConsts &c, int mode, int cfg, const float *px,
     return set_error(NB_ERR_INVALID_ARG, "forward config out of range");
   if (n <= 0) return NB_OK;
   const FwdConfig &fc = kConfigs[cfg];
-  if (fc.scheme == SCHEME_SYMMETRIC) {
+  if (fc.scheme != SCHEME_TARGETS) {
     if (mode != MODE_PLAIN)
       return set_error(NB_ERR_UNSUPPORTED,
                        "symmetric launch shapes take plain accumulation with "
                        "dist_eps^3 a normal float");
-    return launch_forward_sym(c, fc.unroll, fc.split, px, py, pz, vx, vy, vz,
+    return launch_forward_sym(c, fc.unroll, fc.split,
+                              fc.scheme == SCHEME_SYMMETRIC_TILES, px, py, pz,
+                              vx, vy, vz,
                               nx, ny, nz, n, scratch, stream);
   }
   KernelFn fn = find_kernel(fc, mode);

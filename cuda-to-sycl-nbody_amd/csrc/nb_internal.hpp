@@ -43,13 +43,16 @@ enum PairMode : int {
 // Forward decomposition schemes.
 enum FwdScheme : int {
   SCHEME_TARGETS = 0,   // each thread sums all sources for its targets
-  SCHEME_SYMMETRIC = 1, // each pair term evaluated once, applied to both
+  SCHEME_SYMMETRIC = 1, // each pair term evaluated once, applied to both;
+                        // work item = super-tile pair (4 waves)
+  SCHEME_SYMMETRIC_TILES = 2, // the same, work item = tile pair (one wave)
 };
 
 // One compiled launch shape of the forward kernel.  For SCHEME_SYMMETRIC
 // (nb_sym.hip): block = 256 (4 waves), unroll = particles per lane,
 // tile = super-tile length (4 * 64 * unroll), split = super-tiles per work
-// item (row chunk).
+// item (row chunk).  For SCHEME_SYMMETRIC_TILES: tile = tile length
+// (64 * unroll), split = 1.
 struct FwdConfig {
   int block;  // threads per workgroup
   int unroll; // target particles per thread
@@ -66,13 +69,16 @@ int forward_default_config(size_t n, int mode);
 // Largest particle count of the symmetric scheme (its partial buffers grow
 // as n^2: 4.0 GB at 2^20 with 8 particles per lane and 4-tile chunks).
 constexpr size_t kSymMaxParticles = (size_t)1 << 20;
+// Largest particle count of the tile-pair form (0.8 GB of partials at 2^17).
+constexpr size_t kSymTileMaxParticles = (size_t)1 << 17;
 
 // Symmetric scheme (nb_sym.hip; vanishing self term only): scratch bytes
 // for `acc` sums per particle (3 forward and adjoint, 6 fused adjoint) and
 // stream-ordered launches of one forward step / one adjoint step.
-size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc);
+size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc,
+                         bool tiles);
 int launch_adjoint_sym(const StepConsts &c, bool fused, int unroll, int chunk,
-                       const float *const pos[3], const float *const lxn[3],
+                       bool tiles, const float *const pos[3], const float *const lxn[3],
                        const float *const lvn[3], float *const lx[3],
                        float *const lv[3], float *const vel[3],
                        float *const nxt[3], int n, float *scratch,
@@ -100,7 +106,7 @@ struct SymBatchLaunch {
 const void *sym_batch_kernel_fn(int unroll);
 int launch_sym_batch(int unroll, const SymBatchLaunch &L, hipStream_t stream);
 
-int launch_forward_sym(const StepConsts &c, int unroll, int chunk,
+int launch_forward_sym(const StepConsts &c, int unroll, int chunk, bool tiles,
                        const float *px, const float *py, const float *pz,
                        float *vx, float *vy, float *vz, float *nx, float *ny,
                        float *nz, int n, float *scratch, hipStream_t stream);

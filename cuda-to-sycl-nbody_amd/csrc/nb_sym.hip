@@ -37,13 +37,17 @@
 //   * the targets' sums over the chunk are one row partial;
 //   * a finalize kernel sums, per particle, its row partials in chunk order
 //     and then subtracts the reaction partials of its column in GA order,
-//     and applies the rule's epilogue.  The order is fixed: results are deterministic.
+//     and applies the rule's epilogue.  The order is fixed: results are
+//     deterministic.
+// Two more forms reuse the same passes: one tile pair per wave for small n
+// (sym_tile_kernel) and the super-tile pairs of many independent systems on
+// one persistent queue (sym_batch_kernel).
 // Padding (index >= n): ForceRule puts padding particles at 1e30 in every
 // coordinate, so any pair with a real particle has |r|^2 = +inf,
 // rsqrt(inf) = 0 and r * 0 = 0 (exact zeros, no mask).  AdjointRule masks
 // K and t3 of pairs with a padding particle in the items that contain the
-// last, partial super-tile (a product r . h of a far padding particle could
-// overflow).
+// last, partial (super-)tile (a product r . h of a far padding particle
+// could overflow).
 //
 // Per pair term (two interactions): forward 14 VALU + v_rsq_f32 + 3/U DPP
 // moves; adjoint 25 VALU + v_rsq_f32 + v_rcp_f32 + 3/U DPP (fused: 31 VALU,
@@ -86,6 +90,11 @@ __host__ __device__ inline long long row_first(int ga, int ts, int L) {
 // Reaction-partial slot of super-tile pair (ga, gb), ga < gb.
 __host__ __device__ inline long long pair_slot(int ga, int gb, int ts) {
   return (long long)ga * ts - (long long)ga * (ga + 1) / 2 + (gb - ga - 1);
+}
+
+__host__ __device__ inline long long tri_slot(int ga, int gb, int ts) {
+  // row-major index of (ga, gb), ga <= gb, in the upper triangle with diagonal
+  return (long long)ga * ts - (long long)ga * (ga - 1) / 2 + (gb - ga);
 }
 
 struct SymArgs {
@@ -468,6 +477,118 @@ __global__ __launch_bounds__(256) void sym_finalize(SymArgs a) {
   R::finish(a, i, f);
 }
 
+// ---- tile-pair items (small n) ----------------------------------------------
+//
+// Work item = one tile pair (A, B), A <= B, of P = 64 U particles, owned by
+// one wave; a workgroup holds four independent waves (one LDS tile each) and
+// has a single barrier, after staging.  The wave keeps the reaction sums in
+// registers for the whole pair.  Slot tri_slot(A, B, T) holds the row
+// partial [ACC][P] and, for A < B, the negated reaction partial [ACC][P].
+// T (T + 1) / 2 wave items fill the chip at sizes where the super-tile items
+// (T / 4)^2 / 2 workgroups of four waves) do not.
+
+template <class R, int U>
+__global__ __launch_bounds__(256) void sym_tile_kernel(SymArgs a) {
+  constexpr int P = 64 * U;
+  constexpr int ST = kWaves * P;
+  __shared__ float4 src4[R::SRC][ST / 4]; // wave w's source tile is tile w
+  float *src = reinterpret_cast<float *>(src4);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n = a.n, T = a.ts;
+  const long long items = (long long)T * (T + 1) / 2;
+  const long long item = (long long)blockIdx.x * kWaves + w;
+  const bool live = item < items;
+  int A = 0, B = 0;
+  if (live) { // row A: the last row whose first slot is <= item
+    int lo = 0, hi = T - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tri_slot(mid, mid, T) <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    A = lo;
+    B = A + (int)(item - tri_slot(A, A, T));
+  }
+#pragma unroll
+  for (int r = 0; r < U; ++r) {
+    const int o = r * 64 + lane;
+    float v[R::SRC];
+    R::load(a, B * P + o, v);
+    const int d = lds_slot<U>(w * P + o);
+#pragma unroll
+    for (int e = 0; e < R::SRC; ++e) src[e * ST + d] = v[e];
+  }
+  __syncthreads(); // the only barrier: every wave has staged its tile
+  if (!live) return;
+
+  Lane<R, U> L;
+  bool vt[U];
+  const int t0 = A * P + lane * U;
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    R::load(a, t0 + k, L.t[k]);
+    vt[k] = t0 + k < n;
+#pragma unroll
+    for (int e = 0; e < R::ACC; ++e) L.a[k][e] = 0.0f;
+  }
+  float *slot = a.fpart + (size_t)item * (2 * R::ACC * P);
+  bool masked = false;
+  if constexpr (R::MASKED_TAIL) masked = (n % P) != 0 && B == T - 1;
+  if (A == B) {
+    if (masked)
+      tile_pass<R, U, false, R::MASKED_TAIL>(L, src, w, lane, a.c.eps, vt,
+                                             B * P, n);
+    else
+      tile_pass<R, U, false, false>(L, src, w, lane, a.c.eps, vt, B * P, n);
+  } else {
+#pragma unroll
+    for (int q = 0; q < U; ++q)
+#pragma unroll
+      for (int e = 0; e < R::ACC; ++e) L.rs[q][e] = 0.0f;
+    if (masked)
+      tile_pass<R, U, true, R::MASKED_TAIL>(L, src, w, lane, a.c.eps, vt,
+                                            B * P, n);
+    else
+      tile_pass<R, U, true, false>(L, src, w, lane, a.c.eps, vt, B * P, n);
+    // after 64 rotations lane l holds the sums of source set l again
+    float *sp = slot + R::ACC * P + lane * U;
+#pragma unroll
+    for (int q = 0; q < U; ++q)
+#pragma unroll
+      for (int e = 0; e < R::ACC; ++e) sp[e * P + q] = L.rs[q][e];
+  }
+  float *fp = slot + lane * U;
+#pragma unroll
+  for (int k = 0; k < U; ++k)
+#pragma unroll
+    for (int e = 0; e < R::ACC; ++e) fp[e * P + k] = L.a[k][e];
+}
+
+// Per particle: row partials of its tile (B order) minus the negated
+// reaction partials of its column (A order), then the rule's epilogue.
+template <class R, int U>
+__global__ __launch_bounds__(256) void sym_tile_finalize(SymArgs a) {
+  constexpr int P = 64 * U;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  const int g = i / P, o = i % P, T = a.ts;
+  float f[R::ACC];
+#pragma unroll
+  for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
+  for (int b = g; b < T; ++b) {
+    const float *p = a.fpart + (size_t)tri_slot(g, b, T) * (2 * R::ACC * P) + o;
+#pragma unroll
+    for (int e = 0; e < R::ACC; ++e) f[e] = f[e] + p[e * P];
+  }
+  for (int c = 0; c < g; ++c) {
+    const float *p = a.fpart + (size_t)tri_slot(c, g, T) * (2 * R::ACC * P) +
+                     R::ACC * P + o;
+#pragma unroll
+    for (int e = 0; e < R::ACC; ++e) f[e] = f[e] - p[e * P];
+  }
+  R::finish(a, i, f);
+}
+
 // ---- many independent systems (batched persistent launch) ----------------
 //
 // Work item = (system s, super-tile pair ga <= gb of s, slot); the items of
@@ -519,11 +640,6 @@ __device__ __forceinline__ SymArgs system_view(const SymBatchArgs &b, int s,
   a.c.dt = pr.z;
   a.c.G = pr.w;
   return a;
-}
-
-__host__ __device__ inline long long tri_slot(int ga, int gb, int ts) {
-  // row-major index of (ga, gb), ga <= gb, in the upper triangle with diagonal
-  return (long long)ga * ts - (long long)ga * (ga - 1) / 2 + (gb - ga);
 }
 
 template <int U>
@@ -605,36 +721,56 @@ struct SymShape {
 };
 
 template <class R>
-const SymShape *find_shape(int unroll) {
+const SymShape *find_shape(int unroll, bool tiles) {
   static const SymShape shapes[] = {
       {4, sym_kernel<R, 4>, sym_finalize<R, 4>},
       {8, sym_kernel<R, 8>, sym_finalize<R, 8>},
   };
+  static const SymShape tile_shapes[] = {
+      {4, sym_tile_kernel<R, 4>, sym_tile_finalize<R, 4>},
+  };
+  if (tiles) {
+    for (const SymShape &s : tile_shapes)
+      if (s.unroll == unroll) return &s;
+    return nullptr;
+  }
   for (const SymShape &s : shapes)
     if (s.unroll == unroll) return &s;
   return nullptr;
 }
 
 int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
-               float *scratch, int acc, hipStream_t stream) {
-  if ((size_t)n > kSymMaxParticles)
+               float *scratch, int acc, bool tiles, hipStream_t stream) {
+  if ((size_t)n > (tiles ? kSymTileMaxParticles : kSymMaxParticles))
     return set_error(NB_ERR_UNSUPPORTED,
-                     "symmetric launch shapes take at most 2^20 particles");
+                     tiles ? "symmetric tile-pair shapes take at most 2^17 particles"
+                           : "symmetric launch shapes take at most 2^20 particles");
   if (!sh || chunk < 1)
     return set_error(NB_ERR_UNSUPPORTED, "no symmetric kernel for this shape");
   if (!scratch)
     return set_error(NB_ERR_INVALID_ARG, "symmetric launch needs scratch");
-  const int st = kWaves * 64 * unroll;
-  const int ts = (n + st - 1) / st;
-  const long long items = items_upto(ts, chunk);
-  a.fpart = scratch;
-  a.spart = scratch + (size_t)items * acc * st;
+  long long blocks;
+  if (tiles) {
+    const int P = 64 * unroll;
+    const int T = (n + P - 1) / P;
+    a.fpart = scratch;
+    a.spart = nullptr;
+    a.ts = T;
+    a.chunk = 1;
+    blocks = ((long long)T * (T + 1) / 2 + kWaves - 1) / kWaves;
+  } else {
+    const int st = kWaves * 64 * unroll;
+    const int ts = (n + st - 1) / st;
+    blocks = items_upto(ts, chunk);
+    a.fpart = scratch;
+    a.spart = scratch + (size_t)blocks * acc * st;
+    a.ts = ts;
+    a.chunk = chunk;
+  }
   a.n = n;
-  a.ts = ts;
-  a.chunk = chunk;
   void *args[] = {&a};
   int r = hip_check(hipLaunchKernel(reinterpret_cast<const void *>(sh->pairs),
-                                    dim3((unsigned)items), dim3(256), args, 0,
+                                    dim3((unsigned)blocks), dim3(256), args, 0,
                                     stream),
                     "symmetric pair launch");
   if (r) return r;
@@ -646,7 +782,14 @@ int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
 
 } // namespace
 
-size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc) {
+size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc,
+                         bool tiles) {
+  if (tiles) {
+    if (n == 0 || n > kSymTileMaxParticles) return 0;
+    const long long P = 64LL * unroll;
+    const long long T = ((long long)n + P - 1) / P;
+    return sizeof(float) * (size_t)(T * (T + 1) / 2) * 2 * acc * (size_t)P;
+  }
   if (n == 0 || n > kSymMaxParticles || chunk < 1) return 0;
   const long long st = (long long)kWaves * 64 * unroll;
   const long long ts = ((long long)n + st - 1) / st;
@@ -733,7 +876,7 @@ int launch_sym_batch(int unroll, const SymBatchLaunch &L, hipStream_t stream) {
                    "symmetric batch finalize launch");
 }
 
-int launch_forward_sym(const StepConsts &c, int unroll, int chunk,
+int launch_forward_sym(const StepConsts &c, int unroll, int chunk, bool tiles,
                        const float *px, const float *py, const float *pz,
                        float *vx, float *vy, float *vz, float *nx, float *ny,
                        float *nz, int n, float *scratch, hipStream_t stream) {
@@ -749,12 +892,12 @@ int launch_forward_sym(const StepConsts &c, int unroll, int chunk,
   a.ny = ny;
   a.nz = nz;
   a.c = c;
-  return launch_sym(find_shape<ForceRule>(unroll), a, unroll, chunk, n,
-                    scratch, ForceRule::ACC, stream);
+  return launch_sym(find_shape<ForceRule>(unroll, tiles), a, unroll, chunk, n,
+                    scratch, ForceRule::ACC, tiles, stream);
 }
 
 int launch_adjoint_sym(const StepConsts &c, bool fused, int unroll, int chunk,
-                       const float *const pos[3], const float *const lxn[3],
+                       bool tiles, const float *const pos[3], const float *const lxn[3],
                        const float *const lvn[3], float *const lx[3],
                        float *const lv[3], float *const vel[3],
                        float *const nxt[3], int n, float *scratch,
@@ -778,11 +921,11 @@ int launch_adjoint_sym(const StepConsts &c, bool fused, int unroll, int chunk,
     a.nx = nxt[0];
     a.ny = nxt[1];
     a.nz = nxt[2];
-    return launch_sym(find_shape<AdjointRule<true>>(unroll), a, unroll, chunk,
-                      n, scratch, AdjointRule<true>::ACC, stream);
+    return launch_sym(find_shape<AdjointRule<true>>(unroll, tiles), a, unroll,
+                      chunk, n, scratch, AdjointRule<true>::ACC, tiles, stream);
   }
-  return launch_sym(find_shape<AdjointRule<false>>(unroll), a, unroll, chunk,
-                    n, scratch, AdjointRule<false>::ACC, stream);
+  return launch_sym(find_shape<AdjointRule<false>>(unroll, tiles), a, unroll,
+                    chunk, n, scratch, AdjointRule<false>::ACC, tiles, stream);
 }
 
 } // namespace nb

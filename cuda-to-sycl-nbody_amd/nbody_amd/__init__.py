@@ -84,14 +84,16 @@ def forward_config_count() -> int:
     return int(load().nb_forward_config_count())
 
 
-SCHEMES = {0: "targets", 1: "symmetric"}
+SCHEMES = {0: "targets", 1: "symmetric", 2: "symmetric_tiles"}
 
 
 def forward_config_info(cfg: int) -> dict:
     """Launch shape `cfg`: block, unroll, tile, split and scheme ("targets":
     each thread sums all sources for its targets; "symmetric": each pair term
     evaluated once and applied to both particles, where tile is the
-    super-tile length and split the super-tiles per work item)."""
+    super-tile length and split the super-tiles per work item;
+    "symmetric_tiles": the same with one tile pair per wave, tile = the tile
+    length)."""
     lib = load()
     vals = [ctypes.c_int() for _ in range(4)]
     check(lib.nb_forward_config_info(cfg, *[ctypes.byref(v) for v in vals]))

@@ -63,7 +63,7 @@ def adjoint_config_info(cfg: int) -> dict:
     scheme = int(lib.nb_adjoint_config_scheme(cfg))
     if scheme < 0:
         check(NB_ERR_INVALID_ARG)
-    info["scheme"] = {0: "targets", 1: "symmetric"}[scheme]
+    info["scheme"] = {0: "targets", 1: "symmetric", 2: "symmetric_tiles"}[scheme]
     return info
 
 

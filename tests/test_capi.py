@@ -172,8 +172,10 @@ def test_forward_configs_are_described(nb):
         schemes.add(info["scheme"])
         if info["scheme"] == "symmetric":  # 4 waves, super-tile = 4 x 64 x unroll
             assert info["block"] == 256 and info["tile"] == 256 * info["unroll"]
+        if info["scheme"] == "symmetric_tiles":  # one tile of 64 x unroll per wave
+            assert info["block"] == 256 and info["tile"] == 64 * info["unroll"]
     assert nb.forward_config_info(0)["scheme"] == "targets"
-    assert schemes == {"targets", "symmetric"}
+    assert schemes == {"targets", "symmetric", "symmetric_tiles"}
     lib = nb.load()
     assert lib.nb_forward_config_scheme(n) == -1 and lib.nb_forward_config_scheme(-1) == -1
 
@@ -184,7 +186,7 @@ def test_adjoint_configs_are_described(nb):
     assert schemes[0] == "targets" and "symmetric" in schemes
     for c in range(n):  # every symmetric adjoint shape has a forward twin
         info = nb.adjoint_config_info(c)
-        if info["scheme"] == "symmetric":
+        if info["scheme"] != "targets":
             assert any(nb.forward_config_info(f) == info
                        for f in range(nb.forward_config_count()))
     assert nb.load().nb_adjoint_config_scheme(n) == -1

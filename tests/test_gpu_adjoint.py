@@ -31,7 +31,7 @@ def rel(a, b):
 
 def sym_adjoint_configs(nb):
     return [c for c in range(nb.adjoint_config_count())
-            if nb.adjoint_config_info(c)["scheme"] == "symmetric"]
+            if nb.adjoint_config_info(c)["scheme"] in ("symmetric", "symmetric_tiles")]
 
 
 def matching_forward_config(nb, acfg):
@@ -44,7 +44,7 @@ def matching_forward_config(nb, acfg):
     raise AssertionError(f"no forward shape like {a}")
 
 
-@pytest.mark.parametrize("cfg", ["default", "sym0", "sym1", "sym2"])
+@pytest.mark.parametrize("cfg", ["default", "sym0", "sym1", "sym2", "sym3"])
 def test_adjoint_pair_sum_vs_fp64(nb, orc, gpu, cfg):
     """With dt = G = 1 and lam_x' = 0 the kernel's lam_x is the pair sum
     J^T g itself; compare it per particle with the fp64 sum (3000 particles:
@@ -100,14 +100,15 @@ def test_adjoint_step_default_params(nb, orc, gpu, n):
     assert rel(outs[1].cpu().numpy(), np.stack(ref_lv)) < 1e-6
 
 
-@pytest.mark.parametrize("shape", ["targets", "symmetric"])
+@pytest.mark.parametrize("shape", ["targets", "symmetric", "symmetric_tiles"])
 @pytest.mark.parametrize("n", [1, 2, 257, 1500, 5000])
 def test_fused_step_equals_separate(nb, gpu, n, shape):
     """Forward part bit-equal to the forward kernel, adjoint part bit-equal
     to the adjoint kernel (same decomposition, same summation order)."""
     import torch
 
-    acfg = 0 if shape == "targets" else sym_adjoint_configs(nb)[0]
+    acfg = 0 if shape == "targets" else next(
+        c for c in sym_adjoint_configs(nb) if nb.adjoint_config_info(c)["scheme"] == shape)
     fcfg = matching_forward_config(nb, acfg)
 
     pos, vel = nb.plummer_ic(n, seed=n, g_mass=1.0)

@@ -88,10 +88,9 @@ def test_one_step_config1(nb, orc, gpu):
     n = 25600  # 100 x 256, the CPU configuration
     pos, vel = nb.disk_ic(n)
     (gp, gv) = gpu_steps(nb, nb.default_params(num_particles=n), pos, vel, 1)
-    rp, rv = orc.step_f32(orc.params(num_particles=n), pos, vel)
-    emax, e99 = rel_err(gp, rp)
-    print(f"N={n} 1 step: max rel {emax:.2e} p99 {e99:.2e}")
-    assert emax <= TOL_STEP
+    p = orc.params(num_particles=n)
+    rp, rv = orc.step_f32(p, pos, vel)
+    assert_step_parity(orc, p, pos, vel, gp, rp, np.arange(n), f"N={n} step 1")
 
 
 def test_tenth_step_and_trajectory_config1(nb, orc, gpu):
@@ -227,8 +226,10 @@ def test_frame_timing(nb, gpu):
 
 
 def sym_configs(nb):
-    return [c for c in range(nb.forward_config_count())
-            if nb.forward_config_info(c)["scheme"] == "symmetric"]
+    """Symmetric shapes: super-tile items first, then the tile-pair form."""
+    info = [nb.forward_config_info(c) for c in range(nb.forward_config_count())]
+    return ([c for c, i in enumerate(info) if i["scheme"] == "symmetric"] +
+            [c for c, i in enumerate(info) if i["scheme"] == "symmetric_tiles"])
 
 
 def target_config(nb, n):
@@ -407,16 +408,22 @@ def test_symmetric_scheme_rejects_other_modes(nb, gpu):
             nb.forward_step(params, dp, dv, dn, cfg=cfg, acc=acc)
     # the simulator's default shape follows the accumulation mode
     sim = nb.Simulator(nb.default_params(num_particles=70000), *nb.disk_ic(70000))
-    assert nb.forward_config_info(sim.config)["scheme"] == "symmetric"
+    assert nb.forward_config_info(sim.config)["scheme"] != "targets"
     sim.set_accumulation(nb.NB_ACC_KAHAN)
     assert nb.forward_config_info(sim.config)["scheme"] == "targets"
     sim.step_sim()
     sim.set_accumulation(nb.NB_ACC_PLAIN)
-    assert nb.forward_config_info(sim.config)["scheme"] == "symmetric"
+    assert nb.forward_config_info(sim.config)["scheme"] != "targets"
     sim.close()
-    # beyond 2^20 particles: refused before any launch
+    # beyond 2^20 particles (2^17 for tile pairs): refused before any launch
     m = (1 << 20) + 1
     big = [torch.zeros(m, device=gpu) for _ in range(9)]
     with pytest.raises(nb.NbError, match="2\\^20"):
         nb.forward_step(nb.default_params(num_particles=m), big[:3], big[3:6], big[6:],
                         cfg=cfg)
+    tiles = next(c for c in sym_configs(nb)
+                 if nb.forward_config_info(c)["scheme"] == "symmetric_tiles")
+    m = (1 << 17) + 1
+    with pytest.raises(nb.NbError, match="2\\^17"):
+        nb.forward_step(nb.default_params(num_particles=m), [t[:m] for t in big[:3]],
+                        [t[:m] for t in big[3:6]], [t[:m] for t in big[6:]], cfg=tiles)
