@@ -244,6 +244,9 @@ def main() -> int:
     pos, vel = nb.disk_ic(n)
     params = nb.default_params(num_particles=n, sim_iterations_per_frame=args.warmup)
     sim = nb.Simulator(params, pos, vel)
+    # direct launches: capturing a frame of `steps` launches as a hipGraph
+    # would happen inside the timed frame
+    sim.set_graph(False)
     if args.cfg >= 0:
         sim.set_config(args.cfg)
     if args.warmup > 0:

@@ -108,6 +108,12 @@ struct nb_sim {
   float *staging = nullptr; // pinned host copy of pos and vel (6 arrays)
   hipEvent_t ev_stage = nullptr;
   bool staged = false;
+  // Frames of two or more steps replay a captured hipGraph, one per start
+  // buffer, valid for the (steps, shape, pair mode, scratch) they hold.
+  bool use_graph = true;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int graph_iters = 0, graph_cfg = -1, graph_mode = -1;
+  const float *graph_scratch = nullptr;
 };
 
 namespace {
@@ -135,9 +141,18 @@ StepConsts consts_of(const nb_params *p) {
   return c;
 }
 
+void drop_sim_graphs(nb_sim *s) {
+  for (hipGraphExec_t &g : s->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
 void destroy_sim(nb_sim *s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  drop_sim_graphs(s);
   if (s->mem) (void)hipFree(s->mem);
   if (s->scratch) (void)hipFree(s->scratch);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -162,6 +177,21 @@ int ensure_sim_scratch(nb_sim *s) {
   if (st) return st;
   s->scratch = static_cast<float *>(p);
   s->scratch_bytes = bytes;
+  return NB_OK;
+}
+
+// `iters` stream-ordered steps launched one by one, ping-ponging the
+// position buffers (src/simulator.cu:65).
+int launch_steps(nb_sim *s, int iters, int mode) {
+  const StepConsts c = consts_of(&s->p);
+  for (int k = 0; k < iters; ++k) {
+    int st = launch_forward(c, mode, s->cfg, s->pos[0], s->pos[1], s->pos[2],
+                            s->vel[0], s->vel[1], s->vel[2], s->nxt[0],
+                            s->nxt[1], s->nxt[2], (int)s->n, s->scratch,
+                            s->stream);
+    if (st) return st;
+    for (int d = 0; d < 3; ++d) std::swap(s->pos[d], s->nxt[d]);
+  }
   return NB_OK;
 }
 
@@ -286,17 +316,58 @@ int nb_sim_advance(nb_sim *s, int iters) {
   if (!s) return set_error(NB_ERR_INVALID_ARG, "sim is NULL");
   if (iters < 0) return set_error(NB_ERR_INVALID_ARG, "iters < 0");
   if (s->n == 0) return NB_OK;
-  const StepConsts c = consts_of(&s->p);
   const int mode = pair_mode(s->p.calc_method, s->acc, s->p.dist_eps);
-  for (int k = 0; k < iters; ++k) {
-    int st = launch_forward(c, mode, s->cfg, s->pos[0], s->pos[1], s->pos[2],
-                            s->vel[0], s->vel[1], s->vel[2], s->nxt[0],
-                            s->nxt[1], s->nxt[2], (int)s->n, s->scratch,
-                            s->stream);
+  if (!s->use_graph || iters < 2) return launch_steps(s, iters, mode);
+  if (s->graph_iters != iters || s->graph_cfg != s->cfg ||
+      s->graph_mode != mode || s->graph_scratch != s->scratch) {
+    int st = hip_check(hipStreamSynchronize(s->stream), "stream synchronize");
     if (st) return st;
-    // ping-pong of the position buffers (src/simulator.cu:65)
-    for (int d = 0; d < 3; ++d) std::swap(s->pos[d], s->nxt[d]);
+    drop_sim_graphs(s);
+    s->graph_iters = iters;
+    s->graph_cfg = s->cfg;
+    s->graph_mode = mode;
+    s->graph_scratch = s->scratch;
   }
+  const int from = s->pos[0] == s->mem ? 0 : 1;
+  if (!s->graph[from]) {
+    float *pos[3], *nxt[3];
+    for (int d = 0; d < 3; ++d) {
+      pos[d] = s->pos[d];
+      nxt[d] = s->nxt[d];
+    }
+    int st = hip_check(hipStreamBeginCapture(s->stream,
+                                             hipStreamCaptureModeThreadLocal),
+                       "hipStreamBeginCapture");
+    if (st) return st;
+    const int st_launch = launch_steps(s, iters, mode);
+    for (int d = 0; d < 3; ++d) { // capturing advances nothing
+      s->pos[d] = pos[d];
+      s->nxt[d] = nxt[d];
+    }
+    hipGraph_t g = nullptr;
+    st = hip_check(hipStreamEndCapture(s->stream, &g), "hipStreamEndCapture");
+    if (st_launch || st) {
+      if (g) (void)hipGraphDestroy(g);
+      return st_launch ? st_launch : st;
+    }
+    st = hip_check(hipGraphInstantiate(&s->graph[from], g, nullptr, nullptr, 0),
+                   "hipGraphInstantiate");
+    (void)hipGraphDestroy(g);
+    if (st) return st;
+  }
+  int st = hip_check(hipGraphLaunch(s->graph[from], s->stream), "hipGraphLaunch");
+  if (st) return st;
+  if (iters & 1)
+    for (int d = 0; d < 3; ++d) std::swap(s->pos[d], s->nxt[d]);
+  return NB_OK;
+}
+
+int nb_sim_set_graph(nb_sim *s, int enable) {
+  if (!s) return set_error(NB_ERR_INVALID_ARG, "sim is NULL");
+  int st = hip_check(hipStreamSynchronize(s->stream), "stream synchronize");
+  if (st) return st;
+  s->use_graph = enable != 0;
+  if (!s->use_graph) drop_sim_graphs(s);
   return NB_OK;
 }
 

@@ -149,6 +149,11 @@ class Simulator:
     def set_config(self, cfg: int) -> None:
         check(self._lib.nb_sim_set_config(self._h, cfg))
 
+    def set_graph(self, enable: bool) -> None:
+        """Replay frames of two or more steps from a captured hipGraph (the
+        default) or launch every step directly."""
+        check(self._lib.nb_sim_set_graph(self._h, int(bool(enable))))
+
     @property
     def config(self) -> int:
         return int(self._lib.nb_sim_config(self._h))

@@ -112,6 +112,7 @@ def load() -> ctypes.CDLL:
     _sig(lib, "nb_sim_set_accumulation", c_int, [vp, c_int])
     _sig(lib, "nb_sim_set_frame_iterations", c_int, [vp, c_int])
     _sig(lib, "nb_sim_set_config", c_int, [vp, c_int])
+    _sig(lib, "nb_sim_set_graph", c_int, [vp, c_int])
     _sig(lib, "nb_sim_config", c_int, [vp])
     _sig(lib, "nb_sim_stage_readback", c_int, [vp])
     _sig(lib, "nb_sim_staged", c_int, [vp, POINTER(c_void_p)])

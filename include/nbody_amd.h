@@ -160,6 +160,13 @@ int nb_sim_set_accumulation(nb_sim *s, int mode);
 /* Changes simIterationsPerFrame for later nb_sim_step calls. */
 int nb_sim_set_frame_iterations(nb_sim *s, int iters);
 
+/* Frames of two or more steps (nb_sim_step, nb_sim_advance) are captured once
+ * as a hipGraph per (steps, launch shape, accumulation, start buffer) and
+ * replayed; enable = 0 launches every step directly (results are the same
+ * bits).  No counterpart in the reference, whose stepSim launches each step
+ * (src/simulator.cu:59-66).  Default: on. */
+int nb_sim_set_graph(nb_sim *s, int enable);
+
 /* Selects a forward launch shape (see nb_forward_config_*) for later steps;
  * -1 restores the library's choice. */
 int nb_sim_set_config(nb_sim *s, int cfg);
