@@ -264,10 +264,11 @@ struct Lane {
   float rs[U][R::ACC]; // reaction sums of the current source set
 };
 
-// 64 steps of one wave against source tile j of the LDS super-tile
-// (src: R::SRC arrays of ST floats in lds_slot order).  With SYM the reaction sums rotate
-// through the wave.  With MASK, pairs with a padding particle contribute
-// nothing (vt: target validity; sources are valid below index n).
+// 64 steps of one wave against source tile j of the LDS super-tile (src:
+// R::SRC arrays of ST floats in lds_slot order; src0 = global index of the
+// tile's first particle).  With SYM the reaction sums rotate through the
+// wave.  With MASK, pairs with a padding particle contribute nothing (vt:
+// target validity; sources are valid below index n).
 template <class R, int U, bool SYM, bool MASK>
 __device__ __forceinline__ void tile_pass(Lane<R, U> &L, const float *src,
                                           int j, int lane, float eps,
