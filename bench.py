@@ -4,14 +4,19 @@
 
 Workload: the plain forward step at N = 262144 (1024 x 256) in fp32, disk
 initial conditions of the reference generator (std::mt19937 seed 5489),
-SimParam defaults.  One step = one launch of the pair kernel over all N
-targets and N sources (N^2 interactions, 20 FLOP credited per interaction).
+SimParam defaults.  One step = the library's launch shape for N (the
+symmetric scheme: each pair term evaluated once for both particles, then an
+ordered finalize that integrates) over all N targets and N sources: N^2
+interactions, 20 FLOP credited per interaction (--cfg picks another shape).
 
 Prints one JSON line.  `value` is Ginteractions/s of the timed steps with
-the particles resident in HBM.  `roofline` prices the pair kernel against
-MI355X's fp32 vector peak (157.3 TFLOP/s) using HIP events on the launch
-stream.  `cpu_baseline` times the fp32 restatement of the reference kernel
-(oracle, OpenMP) on a bounded sample of the same step on the host cores.
+the particles resident in HBM.  `roofline` prices the step's kernels
+against MI355X's fp32 peak (157.3 TFLOP/s) using HIP events on the launch
+stream; `executed_frac` prices the FLOP the hardware executes instead of the
+credited ones; `traffic` is the pair kernel's HBM bytes per launch from the
+committed rocprofv3 counter summary.  `cpu_baseline` times the fp32
+restatement of the reference kernel (oracle, OpenMP) on a bounded sample of
+the same step on the host cores.
 
 Also in the line (same run, same N):
   fwd_adjoint  the fused forward+adjoint step (one launch, N^2 pairs) and the

@@ -283,6 +283,7 @@ const FwdConfig kAdjConfigs[] = {
     {256, 4, 1024, 1, SCHEME_SYMMETRIC}, {256, 4, 1024, 2, SCHEME_SYMMETRIC},
     {256, 8, 2048, 1, SCHEME_SYMMETRIC},
     {256, 4, 256, 1, SCHEME_SYMMETRIC_TILES},
+    {256, 2, 128, 1, SCHEME_SYMMETRIC_TILES},
 };
 constexpr int kNumAdj = sizeof(kAdjConfigs) / sizeof(kAdjConfigs[0]);
 
@@ -300,14 +301,17 @@ const FwdConfig &adjoint_config(int cfg) { return kAdjConfigs[cfg]; }
 
 int adjoint_default_config(size_t n, bool skip) {
   // Measured on MI355X (disk ICs), adjoint / fused ms:
-  //   n = 2048: 256 x 1 split 2 0.066 / 0.071, tile pairs 0.066 / 0.079;
-  //   n = 4096 .. 65536: tile pairs 0.069 / 0.082 .. 1.72 / 2.17, super-tile
-  //     pairs 0.23 / 0.28 .. 1.84 / 2.20, best target-sharded 0.11 / 0.12 ..
-  //     2.69 / 3.09;
-  //   n = 262144: super-tile pairs 25.5 / 31.3, target-sharded 38.7 / 43.5.
+  //   tile pairs, 2 per lane: n = 2048 0.030 / 0.037, 4096 0.033 / 0.039,
+  //     16384 0.165 / 0.189, 32768 0.486 / 0.568;
+  //   tile pairs, 4 per lane: 2048 0.066 / 0.079, 16384 0.197 / 0.236,
+  //     65536 1.72 / 2.17;
+  //   super-tile pairs: 65536 1.84 / 2.20, 262144 25.5 / 31.3;
+  //   best target-sharded: 2048 0.066 / 0.071, 16384 0.427 / 0.468,
+  //     262144 38.7 / 43.5.
   if (!skip && n <= kSymMaxParticles) {
-    if (n >= ((size_t)1 << 17)) return 14; // symmetric super-tile pairs
-    if (n >= 3072) return 17;              // symmetric tile pairs
+    if (n >= ((size_t)1 << 17)) return 14; // super-tile pairs, 4 per lane
+    if (n >= 49152) return 17;             // tile pairs, 4 per lane
+    if (n >= 1024) return 18;              // tile pairs, 2 per lane
   }
   if (n >= (size_t)1 << 17) return 8; // 256 x 1, split 2
   if (n < 4096) return 8;

@@ -149,6 +149,7 @@ const FwdConfig kConfigs[] = {
     {256, 1, 1024, 16}, {256, 2, 1024, 16}, {128, 1, 512, 8},
     {128, 1, 512, 16},
     {256, 4, 256, 1, SCHEME_SYMMETRIC_TILES},
+    {256, 2, 128, 1, SCHEME_SYMMETRIC_TILES},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 

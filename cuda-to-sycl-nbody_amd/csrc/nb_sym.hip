@@ -249,11 +249,13 @@ struct AdjointRule {
 // holds members 4 qq .. 4 qq + 3 of set m, so the 64 lanes of a step (sets
 // (l + s) mod 64, one quad each) read one contiguous 1 KiB block: no bank
 // conflicts.
+// With U = 2 (tile-pair form only) set m's two members sit in float2 slot m.
 template <int U>
 __device__ __forceinline__ int lds_slot(int o) {
   constexpr int P = 64 * U;
   const int j = o / P, r = o % P;
   const int m = r / U, q = r % U;
+  if constexpr (U == 2) return j * P + m * 2 + q;
   return j * P + (q >> 2) * 256 + m * 4 + (q & 3);
 }
 
@@ -277,19 +279,27 @@ __device__ __forceinline__ void tile_pass(Lane<R, U> &L, const float *src,
   constexpr int P = 64 * U;
   constexpr int ST = kWaves * P;
   constexpr int Q = U / 4;
+  static_assert(U == 2 || U % 4 == 0, "particles per lane: 2 or a multiple of 4");
   for (int s = 0; s < 64; ++s) {
     const int m = (lane + s) & 63;
     float sv[U][R::SRC];
 #pragma unroll
     for (int c = 0; c < R::SRC; ++c) {
-      const float4 *p4 = reinterpret_cast<const float4 *>(src + c * ST + j * P);
+      if constexpr (U == 2) {
+        const float2 v = reinterpret_cast<const float2 *>(src + c * ST + j * P)[m];
+        sv[0][c] = v.x;
+        sv[1][c] = v.y;
+      } else {
+        const float4 *p4 =
+            reinterpret_cast<const float4 *>(src + c * ST + j * P);
 #pragma unroll
-      for (int qq = 0; qq < Q; ++qq) {
-        const float4 v = p4[qq * 64 + m];
-        sv[4 * qq + 0][c] = v.x;
-        sv[4 * qq + 1][c] = v.y;
-        sv[4 * qq + 2][c] = v.z;
-        sv[4 * qq + 3][c] = v.w;
+        for (int qq = 0; qq < Q; ++qq) {
+          const float4 v = p4[qq * 64 + m];
+          sv[4 * qq + 0][c] = v.x;
+          sv[4 * qq + 1][c] = v.y;
+          sv[4 * qq + 2][c] = v.z;
+          sv[4 * qq + 3][c] = v.w;
+        }
       }
     }
 #pragma unroll
@@ -728,6 +738,7 @@ const SymShape *find_shape(int unroll, bool tiles) {
       {8, sym_kernel<R, 8>, sym_finalize<R, 8>},
   };
   static const SymShape tile_shapes[] = {
+      {2, sym_tile_kernel<R, 2>, sym_tile_finalize<R, 2>},
       {4, sym_tile_kernel<R, 4>, sym_tile_finalize<R, 4>},
   };
   if (tiles) {

@@ -168,7 +168,9 @@ def test_forward_configs_are_described(nb):
     schemes = set()
     for c in range(n):
         info = nb.forward_config_info(c)
-        assert info["tile"] % info["block"] == 0 and info["split"] >= 1
+        assert info["split"] >= 1
+        if info["scheme"] == "targets":  # the block stages whole LDS tiles
+            assert info["tile"] % info["block"] == 0
         schemes.add(info["scheme"])
         if info["scheme"] == "symmetric":  # 4 waves, super-tile = 4 x 64 x unroll
             assert info["block"] == 256 and info["tile"] == 256 * info["unroll"]
