@@ -500,7 +500,7 @@ int nb_batch_create(int num_systems, const nb_params *params, const float *px,
     const size_t n = params[s].num_particles;
     scratch_bytes = std::max(
         scratch_bytes,
-        forward_scratch_bytes(forward_default_config(n, b->mode), n));
+        forward_scratch_bytes(forward_default_config(n, b->mode), n, b->mode));
   }
   const size_t floats = 15 * std::max<size_t>(b->stride, 64);
   if ((st = hip_check(hipMalloc(&b->mem, sizeof(float) * floats),

@@ -164,7 +164,10 @@ void destroy_sim(nb_sim *s) {
 }
 
 int ensure_sim_scratch(nb_sim *s) {
-  const size_t bytes = s->n ? forward_scratch_bytes(s->cfg, s->n) : 0;
+  const size_t bytes =
+      s->n ? forward_scratch_bytes(
+                 s->cfg, s->n, pair_mode(s->p.calc_method, s->acc, s->p.dist_eps))
+           : 0;
   if (bytes == s->scratch_bytes) return NB_OK;
   if (s->scratch) {
     (void)hipFree(s->scratch);
@@ -545,7 +548,7 @@ int nb_forward_step_dev(const nb_params *p, const float *px, const float *py,
   const int mode = pair_mode(p->calc_method, acc, p->dist_eps);
   if (cfg < 0) cfg = forward_default_config(n, mode);
   float *scratch = nullptr;
-  const size_t bytes = forward_scratch_bytes(cfg, n);
+  const size_t bytes = forward_scratch_bytes(cfg, n, mode);
   if (bytes) {
     scratch = device_scratch(bytes, &st);
     if (st) return st;

@@ -180,6 +180,17 @@ int forward_default_config(size_t n, int mode) {
     if (n >= ((size_t)1 << 17)) return 22; // symmetric, 8 per lane, 1 per item
     if (n >= 20480) return 32;             // symmetric tile pairs, 4 per lane
   }
+  // Kahan, Gint/s, symmetric (4 per lane) vs target-sharded: n = 32768
+  // tile pairs 1905 vs 1880; 65536 2152 vs 2123; 262144 super-tile pairs
+  // 2445 vs 2379; 2^20 2469 vs 2474.  The compensated sums' dependency
+  // chains, not the pair count, bound this mode; above 2^18 the symmetric
+  // partials ((sum, compensation) pairs) would need gigabytes of scratch.
+  if (mode == MODE_KAHAN) {
+    if (n >= ((size_t)1 << 17) && n <= ((size_t)1 << 18))
+      return 25;               // symmetric, 4 per lane, 1 per item
+    if (n >= 20480 && n < ((size_t)1 << 17))
+      return 32;               // symmetric tile pairs, 4 per lane
+  }
   // Target-sharded, Gint/s: n = 4096: 128 x 1 split 8 629 (256 x 1 split 8
   // 480); 8192: 256 x 1 split 8 1746 (split 16 1629); 16384 .. 32768:
   // 256 x 1 split 16 2843 .. 3432 (split 8 2792 .. 3292); 65536: 256 x 4
@@ -190,10 +201,10 @@ int forward_default_config(size_t n, int mode) {
   return 30;                           // 128 x 1, tile 512, split 8
 }
 
-size_t forward_scratch_bytes(int cfg, size_t n) {
+size_t forward_scratch_bytes(int cfg, size_t n, int mode) {
   const FwdConfig &c = kConfigs[cfg];
-  if (c.scheme != SCHEME_TARGETS)
-    return sym_scratch_bytes(c.unroll, c.split, n, 3,
+  if (c.scheme != SCHEME_TARGETS) // (sum, compensation) pairs for Kahan
+    return sym_scratch_bytes(c.unroll, c.split, n, mode == MODE_KAHAN ? 6 : 3,
                              c.scheme == SCHEME_SYMMETRIC_TILES);
   return c.split > 1 ? sizeof(float) * 3 * (size_t)c.split * n : 0;
 }
@@ -207,14 +218,17 @@ int launch_forward(const StepConsts &c, int mode, int cfg, const float *px,
   if (n <= 0) return NB_OK;
   const FwdConfig &fc = kConfigs[cfg];
   if (fc.scheme != SCHEME_TARGETS) {
-    if (mode != MODE_PLAIN)
+    if (mode != MODE_PLAIN && mode != MODE_KAHAN)
       return set_error(NB_ERR_UNSUPPORTED,
-                       "symmetric launch shapes take plain accumulation with "
-                       "dist_eps^3 a normal float");
+                       "symmetric launch shapes take BRANCH with dist_eps^3 "
+                       "a normal float");
+    if (mode == MODE_KAHAN && fc.unroll != 4)
+      return set_error(NB_ERR_UNSUPPORTED,
+                       "symmetric Kahan shapes have 4 particles per lane");
     return launch_forward_sym(c, fc.unroll, fc.split,
-                              fc.scheme == SCHEME_SYMMETRIC_TILES, px, py, pz,
-                              vx, vy, vz,
-                              nx, ny, nz, n, scratch, stream);
+                              fc.scheme == SCHEME_SYMMETRIC_TILES,
+                              mode == MODE_KAHAN, px, py, pz, vx, vy, vz, nx,
+                              ny, nz, n, scratch, stream);
   }
   KernelFn fn = find_kernel(fc, mode);
   if (!fn) return set_error(NB_ERR_UNSUPPORTED, "no kernel for this mode");

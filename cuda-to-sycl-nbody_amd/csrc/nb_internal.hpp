@@ -107,7 +107,7 @@ const void *sym_batch_kernel_fn(int unroll);
 int launch_sym_batch(int unroll, const SymBatchLaunch &L, hipStream_t stream);
 
 int launch_forward_sym(const StepConsts &c, int unroll, int chunk, bool tiles,
-                       const float *px, const float *py, const float *pz,
+                       bool kahan, const float *px, const float *py, const float *pz,
                        float *vx, float *vy, float *vz, float *nx, float *ny,
                        float *nz, int n, float *scratch, hipStream_t stream);
 
@@ -118,8 +118,8 @@ int launch_forward(const StepConsts &c, int mode, int cfg, const float *px,
                    float *vz, float *nx, float *ny, float *nz, int n,
                    float *scratch, hipStream_t stream);
 
-// Bytes of scratch launch_forward needs for (cfg, n).
-size_t forward_scratch_bytes(int cfg, size_t n);
+// Bytes of scratch launch_forward needs for (cfg, n, pair mode).
+size_t forward_scratch_bytes(int cfg, size_t n, int mode);
 
 // Per-device scratch owned by the library (grown on demand, never shrunk).
 // Calls on different streams of one device share it.

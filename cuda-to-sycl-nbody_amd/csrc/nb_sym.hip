@@ -111,7 +111,24 @@ struct SymArgs {
   StepConsts c;
 };
 
-struct ForceRule {
+// Finalize arithmetic of the plain rules: row partials are added, the
+// negated reaction partials subtracted, in the order the caller visits them.
+template <int ACC>
+struct PlainPartials {
+  static __device__ __forceinline__ void add_row(float (&f)[ACC],
+                                                 const float *p, int stride) {
+#pragma unroll
+    for (int e = 0; e < ACC; ++e) f[e] = f[e] + p[e * stride];
+  }
+  static __device__ __forceinline__ void sub_reaction(float (&f)[ACC],
+                                                      const float *p,
+                                                      int stride) {
+#pragma unroll
+    for (int e = 0; e < ACC; ++e) f[e] = f[e] - p[e * stride];
+  }
+};
+
+struct ForceRule : PlainPartials<3> {
   static constexpr int SRC = 3; // x, y, z
   static constexpr int ACC = 3; // F
   static constexpr bool MASKED_TAIL = false;
@@ -157,8 +174,75 @@ struct ForceRule {
   }
 };
 
+// Kahan-compensated forward rule (NB_ACC_KAHAN): the target sums and the
+// (negated) reaction sums each carry a running compensation c (kahan_add,
+// nb_pairs.hpp: y = fma(r, K, -c), t = s + y, c = (t - s) - y, s = t; the
+// sum is s - c).  A partial is (s_x, s_y, s_z, c_x, c_y, c_z); the finalize
+// Kahan-adds the partials' values s - c.
+struct KahanForceRule {
+  static constexpr int SRC = 3;
+  static constexpr int ACC = 6;
+  static constexpr bool MASKED_TAIL = false;
+
+  static __device__ __forceinline__ void load(const SymArgs &a, int j,
+                                              float (&v)[SRC]) {
+    ForceRule::load(a, j, v);
+  }
+
+  template <bool SYM, bool MASK>
+  static __device__ __forceinline__ void pair(const float (&s)[SRC],
+                                              const float (&t)[SRC],
+                                              float (&a)[ACC],
+                                              float (&rs)[ACC], float eps,
+                                              bool) {
+    const float rx = s[0] - t[0];
+    const float ry = s[1] - t[1];
+    const float rz = s[2] - t[2];
+    float d = __builtin_fmaf(rx, rx, eps);
+    d = __builtin_fmaf(ry, ry, d);
+    d = __builtin_fmaf(rz, rz, d);
+    const float K = __builtin_amdgcn_rsqf((d * d) * d);
+    kahan_add(rx, K, a[0], a[3]);
+    kahan_add(ry, K, a[1], a[4]);
+    kahan_add(rz, K, a[2], a[5]);
+    if constexpr (SYM) {
+      kahan_add(rx, K, rs[0], rs[3]);
+      kahan_add(ry, K, rs[1], rs[4]);
+      kahan_add(rz, K, rs[2], rs[5]);
+    }
+  }
+
+  // f: running (sum, compensation) of each coordinate over the partials
+  static __device__ __forceinline__ void kahan_value(float (&f)[ACC], int e,
+                                                     float v) {
+    const float y = v - f[3 + e];
+    const float t = f[e] + y;
+    f[3 + e] = (t - f[e]) - y;
+    f[e] = t;
+  }
+  static __device__ __forceinline__ void add_row(float (&f)[ACC],
+                                                 const float *p, int stride) {
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+      kahan_value(f, e, p[e * stride] - p[(3 + e) * stride]);
+  }
+  static __device__ __forceinline__ void sub_reaction(float (&f)[ACC],
+                                                      const float *p,
+                                                      int stride) {
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+      kahan_value(f, e, p[(3 + e) * stride] - p[e * stride]);
+  }
+
+  static __device__ __forceinline__ void finish(const SymArgs &a, int i,
+                                                const float (&f)[ACC]) {
+    integrate_ref(a.c, i, f[0] - f[3], f[1] - f[4], f[2] - f[5], a.px, a.py,
+                  a.pz, a.vx, a.vy, a.vz, a.nx, a.ny, a.nz);
+  }
+};
+
 template <bool FUSED>
-struct AdjointRule {
+struct AdjointRule : PlainPartials<FUSED ? 6 : 3> {
   static constexpr int SRC = 6; // x, y, z, g_x, g_y, g_z
   static constexpr int ACC = FUSED ? 6 : 3; // sum M (g_j - g_k) [, F]
   static constexpr bool MASKED_TAIL = true;
@@ -475,16 +559,11 @@ __global__ __launch_bounds__(256) void sym_finalize(SymArgs a) {
   float f[R::ACC];
 #pragma unroll
   for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
-  for (int c = 0; c < nc; ++c) {
-    const float *p = a.fpart + (size_t)(first + c) * (R::ACC * ST) + o;
-#pragma unroll
-    for (int e = 0; e < R::ACC; ++e) f[e] = f[e] + p[e * ST];
-  }
-  for (int ga = 0; ga < g; ++ga) { // negated reaction sums
-    const float *p = a.spart + (size_t)pair_slot(ga, g, ts) * (R::ACC * ST) + o;
-#pragma unroll
-    for (int e = 0; e < R::ACC; ++e) f[e] = f[e] - p[e * ST];
-  }
+  for (int c = 0; c < nc; ++c)
+    R::add_row(f, a.fpart + (size_t)(first + c) * (R::ACC * ST) + o, ST);
+  for (int ga = 0; ga < g; ++ga) // negated reaction sums
+    R::sub_reaction(
+        f, a.spart + (size_t)pair_slot(ga, g, ts) * (R::ACC * ST) + o, ST);
   R::finish(a, i, f);
 }
 
@@ -586,17 +665,14 @@ __global__ __launch_bounds__(256) void sym_tile_finalize(SymArgs a) {
   float f[R::ACC];
 #pragma unroll
   for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
-  for (int b = g; b < T; ++b) {
-    const float *p = a.fpart + (size_t)tri_slot(g, b, T) * (2 * R::ACC * P) + o;
-#pragma unroll
-    for (int e = 0; e < R::ACC; ++e) f[e] = f[e] + p[e * P];
-  }
-  for (int c = 0; c < g; ++c) {
-    const float *p = a.fpart + (size_t)tri_slot(c, g, T) * (2 * R::ACC * P) +
-                     R::ACC * P + o;
-#pragma unroll
-    for (int e = 0; e < R::ACC; ++e) f[e] = f[e] - p[e * P];
-  }
+  for (int b = g; b < T; ++b)
+    R::add_row(f, a.fpart + (size_t)tri_slot(g, b, T) * (2 * R::ACC * P) + o,
+               P);
+  for (int c = 0; c < g; ++c)
+    R::sub_reaction(f,
+                    a.fpart + (size_t)tri_slot(c, g, T) * (2 * R::ACC * P) +
+                        R::ACC * P + o,
+                    P);
   R::finish(a, i, f);
 }
 
@@ -710,17 +786,14 @@ __global__ __launch_bounds__(256) void sym_batch_finalize(SymBatchArgs b) {
   float f[R::ACC];
 #pragma unroll
   for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
-  for (int gb = g; gb < a.ts; ++gb) {
-    const float *p = base + (size_t)tri_slot(g, gb, a.ts) * (2 * R::ACC * ST);
-#pragma unroll
-    for (int e = 0; e < R::ACC; ++e) f[e] = f[e] + p[e * ST];
-  }
-  for (int ga = 0; ga < g; ++ga) { // negated reaction sums
-    const float *p = base + (size_t)tri_slot(ga, g, a.ts) * (2 * R::ACC * ST) +
-                     R::ACC * ST;
-#pragma unroll
-    for (int e = 0; e < R::ACC; ++e) f[e] = f[e] - p[e * ST];
-  }
+  for (int gb = g; gb < a.ts; ++gb)
+    R::add_row(f, base + (size_t)tri_slot(g, gb, a.ts) * (2 * R::ACC * ST),
+               ST);
+  for (int ga = 0; ga < g; ++ga) // negated reaction sums
+    R::sub_reaction(f,
+                    base + (size_t)tri_slot(ga, g, a.ts) * (2 * R::ACC * ST) +
+                        R::ACC * ST,
+                    ST);
   R::finish(a, li, f);
 }
 
@@ -749,6 +822,16 @@ const SymShape *find_shape(int unroll, bool tiles) {
   for (const SymShape &s : shapes)
     if (s.unroll == unroll) return &s;
   return nullptr;
+}
+
+// Kahan shapes: 4 per lane only (the compensations double the sums).
+const SymShape *find_kahan_shape(int unroll, bool tiles) {
+  static const SymShape super4 = {4, sym_kernel<KahanForceRule, 4>,
+                                  sym_finalize<KahanForceRule, 4>};
+  static const SymShape tiles4 = {4, sym_tile_kernel<KahanForceRule, 4>,
+                                  sym_tile_finalize<KahanForceRule, 4>};
+  if (unroll != 4) return nullptr;
+  return tiles ? &tiles4 : &super4;
 }
 
 int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
@@ -889,9 +972,10 @@ int launch_sym_batch(int unroll, const SymBatchLaunch &L, hipStream_t stream) {
 }
 
 int launch_forward_sym(const StepConsts &c, int unroll, int chunk, bool tiles,
-                       const float *px, const float *py, const float *pz,
-                       float *vx, float *vy, float *vz, float *nx, float *ny,
-                       float *nz, int n, float *scratch, hipStream_t stream) {
+                       bool kahan, const float *px, const float *py,
+                       const float *pz, float *vx, float *vy, float *vz,
+                       float *nx, float *ny, float *nz, int n, float *scratch,
+                       hipStream_t stream) {
   if (n <= 0) return NB_OK;
   SymArgs a = {};
   a.px = px;
@@ -904,6 +988,9 @@ int launch_forward_sym(const StepConsts &c, int unroll, int chunk, bool tiles,
   a.ny = ny;
   a.nz = nz;
   a.c = c;
+  if (kahan)
+    return launch_sym(find_kahan_shape(unroll, tiles), a, unroll, chunk, n,
+                      scratch, KahanForceRule::ACC, tiles, stream);
   return launch_sym(find_shape<ForceRule>(unroll, tiles), a, unroll, chunk, n,
                     scratch, ForceRule::ACC, tiles, stream);
 }

@@ -98,8 +98,8 @@ TrajLayout traj_layout(size_t n, int every) {
   const int cs = forward_default_config(n, MODE_PLAIN_SKIP);
   const int ca = adjoint_default_config(n, false);
   const int ck = adjoint_default_config(n, true);
-  const size_t sb = std::max({forward_scratch_bytes(cf, n),
-                              forward_scratch_bytes(cs, n),
+  const size_t sb = std::max({forward_scratch_bytes(cf, n, MODE_PLAIN),
+                              forward_scratch_bytes(cs, n, MODE_PLAIN_SKIP),
                               adjoint_scratch_bytes(ca, n),
                               adjoint_scratch_bytes(ck, n)});
   L.total = L.scratch + (sb + sizeof(float) - 1) / sizeof(float);

@@ -399,7 +399,8 @@ def test_symmetric_scheme_rejects_other_modes(nb, gpu):
     cfg = sym_configs(nb)[0]
     n = 3000
     pos, vel = nb.disk_ic(n)
-    for kw, acc in (({}, nb.NB_ACC_KAHAN), ({"dist_eps": 0.0}, nb.NB_ACC_PLAIN),
+    for kw, acc in (({"dist_eps": 0.0}, nb.NB_ACC_PLAIN),
+                    ({"dist_eps": 0.0}, nb.NB_ACC_KAHAN),
                     ({"calc_method": nb.NB_CALC_PREDICATED}, nb.NB_ACC_PLAIN)):
         params = nb.default_params(num_particles=n, **kw)
         dp, dv = dev3(torch, pos, gpu), dev3(torch, vel, gpu)
@@ -409,9 +410,13 @@ def test_symmetric_scheme_rejects_other_modes(nb, gpu):
     # the simulator's default shape follows the accumulation mode
     sim = nb.Simulator(nb.default_params(num_particles=70000), *nb.disk_ic(70000))
     assert nb.forward_config_info(sim.config)["scheme"] != "targets"
-    sim.set_accumulation(nb.NB_ACC_KAHAN)
-    assert nb.forward_config_info(sim.config)["scheme"] == "targets"
+    sim.set_accumulation(nb.NB_ACC_KAHAN)  # the symmetric Kahan shapes: 4 per lane
+    assert nb.forward_config_info(sim.config)["unroll"] == 4
     sim.step_sim()
+    sim.set_config(sym_configs(nb)[0])  # 8 per lane: no Kahan variant
+    with pytest.raises(nb.NbError, match="4 particles per lane"):
+        sim.step_sim()
+    sim.set_config(-1)
     sim.set_accumulation(nb.NB_ACC_PLAIN)
     assert nb.forward_config_info(sim.config)["scheme"] != "targets"
     sim.close()
@@ -449,3 +454,44 @@ def test_frame_graph_replays_match_direct_launches(nb, gpu, n):
             assert np.array_equal(a, b), (k, iters)
     g.close()
     d.close()
+
+
+@pytest.mark.parametrize("n", [3000, 70001])
+def test_symmetric_kahan(nb, orc, gpu, n):
+    """Kahan-compensated accumulation in the symmetric shapes (4 per lane):
+    parity with the oracle step, and forces closer to fp64 than the plain
+    symmetric sums (both sampled against the fp64 oracle forces)."""
+    import torch
+
+    pos, vel = nb.disk_ic(n)
+    idx = np.arange(0, n, 7)
+    f64 = np.stack(orc.forces_f64(pos, idx))
+    unit = nb.default_params(num_particles=n, G=1.0, dt=1.0, damping=0.0)
+    kahan_cfgs = [c for c in sym_configs(nb) if nb.forward_config_info(c)["unroll"] == 4]
+    assert kahan_cfgs
+    for cfg in kahan_cfgs:
+        errs = {}
+        for acc in (nb.NB_ACC_PLAIN, nb.NB_ACC_KAHAN):
+            dp = dev3(torch, pos, gpu)
+            dv = [torch.zeros(n, device=gpu) for _ in range(3)]
+            dn = [torch.empty_like(t) for t in dp]
+            nb.forward_step(unit, dp, dv, dn, cfg=cfg, acc=acc)  # v' = F
+            torch.cuda.synchronize()
+            F = np.stack([t.cpu().numpy()[idx] for t in dv]).astype(np.float64)
+            errs[acc] = np.abs(F - f64).max(0) / np.abs(f64).max(0)
+        info = nb.forward_config_info(cfg)
+        print(f"N={n} cfg {cfg} {info}: plain p50 {np.median(errs[0]):.2e} "
+              f"max {errs[0].max():.2e}; kahan p50 {np.median(errs[1]):.2e} "
+              f"max {errs[1].max():.2e}")
+        assert np.median(errs[1]) < np.median(errs[0]) and errs[1].max() < 1e-5
+        # one SimParam-default step against the oracle
+        params = nb.default_params(num_particles=n)
+        dp, dv = dev3(torch, pos, gpu), dev3(torch, vel, gpu)
+        dn = [torch.empty_like(t) for t in dp]
+        nb.forward_step(params, dp, dv, dn, cfg=cfg, acc=nb.NB_ACC_KAHAN)
+        torch.cuda.synchronize()
+        p = orc.params(num_particles=n)
+        rp, _ = orc.step_f32(p, pos, vel, 0, min(n, 3000))
+        sl = slice(0, min(n, 3000))
+        assert_step_parity(orc, p, pos, vel, [t.cpu().numpy()[sl] for t in dn],
+                           [a[sl] for a in rp], np.arange(sl.stop), f"kahan N={n} cfg {cfg}")
