@@ -2,6 +2,7 @@
 
     python -m nbody_amd.studies precision [--n 1048576] [--steps 100]
     python -m nbody_amd.studies adjoint   [--n 16384] [--steps 200] [--every 20]
+    python -m nbody_amd.studies shapes    [--sizes 16384 65536 262144]
 
 precision: N particles, half a Plummer sphere (scale radius 1, virial
   velocities for the simulation's own G and unit masses, centred on the disk
@@ -16,6 +17,9 @@ adjoint: the reference disk, a target made by a forward run from initial
   from the checkpointed adjoint, compared with central differences
   (h = 1e-3, fp32 GPU forward runs) on 32 coordinates.  (The comparison with
   an fp64 CPU adjoint is tests/adjoint_fp64_check.py.)
+shapes: every compiled launch shape timed at each size (forward step plain
+  and Kahan, adjoint plain and fused), fastest first: the measurements the
+  library's default shapes come from.
 
 Prints one JSON object.
 """
@@ -216,9 +220,78 @@ def adjoint(n: int = 16384, steps: int = 200, every: int = 20, fd_coords: int = 
     return out
 
 
+def shapes(sizes, kinds=("forward", "adjoint", "kahan")) -> dict:
+    """Times every compiled launch shape on the reference disk at each size:
+    the forward step (plain and Kahan accumulation) and the adjoint (plain
+    and fused with the forward step).  Shapes that refuse a size or mode are
+    skipped.  Returns {kind: {n: [(cfg, info, ms), ...]}}, fastest first."""
+    import nbody_amd as nb
+
+    torch = _torch()
+    dev = torch.device("cuda:0")
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {k: {} for k in kinds}
+    for n in sizes:
+        pos, vel = nb.disk_ic(n)
+        params = nb.default_params(num_particles=n)
+        P = [torch.from_numpy(a).to(dev) for a in pos]
+        V = [torch.from_numpy(a).to(dev) for a in vel]
+        N = [torch.empty_like(t) for t in P]
+        reps = max(2, min(20, int(1e12 / (float(n) * n))))
+        for kind in kinds:
+            rows = []
+            if kind in ("forward", "kahan"):
+                acc = nb.NB_ACC_KAHAN if kind == "kahan" else nb.NB_ACC_PLAIN
+                for cfg in range(nb.forward_config_count()):
+                    try:
+                        ms = timed(lambda: nb.forward_step(params, P, V, N, cfg=cfg, acc=acc),
+                                   reps)
+                    except nb.NbError:
+                        continue
+                    rows.append((cfg, nb.forward_config_info(cfg), ms))
+            else:
+                rng = np.random.default_rng(0)
+                lxn = torch.from_numpy(rng.normal(size=(3, n)).astype(np.float32)).to(dev)
+                lvn = torch.from_numpy(rng.normal(size=(3, n)).astype(np.float32)).to(dev)
+                lx, lv = torch.empty_like(lxn), torch.empty_like(lxn)
+                P3 = torch.stack(P)
+                V3 = torch.stack(V)
+                N3 = torch.empty_like(P3)
+                for cfg in range(nb.adjoint_config_count()):
+                    try:
+                        ms = timed(lambda: nb.adjoint_step(params, P3, lxn, lvn, lx, lv,
+                                                           cfg=cfg), reps)
+                        fused = timed(lambda: nb.fwd_adjoint_step(params, P3, V3, N3, lxn, lvn,
+                                                                  lx, lv, cfg=cfg), reps)
+                    except nb.NbError:
+                        continue
+                    rows.append((cfg, nb.adjoint_config_info(cfg), ms, fused))
+            rows.sort(key=lambda r: r[2])
+            out[kind][n] = rows
+            best = rows[0] if rows else None
+            print(f"shapes: {kind} N={n}: {len(rows)} shapes, fastest cfg "
+                  f"{best[0] if best else None} {best[2] if best else 0:.4f} ms",
+                  file=sys.stderr, flush=True)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["precision", "adjoint"])
+    ap.add_argument("study", choices=["precision", "adjoint", "shapes"])
+    ap.add_argument("--sizes", type=int, nargs="+", default=[16384, 65536, 262144],
+                    help="shapes: particle counts")
     ap.add_argument("--n", type=int)
     ap.add_argument("--steps", type=int)
     ap.add_argument("--every", type=int, default=20)
@@ -229,7 +302,9 @@ def main(argv=None) -> int:
     ap.add_argument("--damping", type=float, help="precision: damping (default: SimParam's)")
     a = ap.parse_args(argv)
     _lib.load()
-    if a.study == "precision":
+    if a.study == "shapes":
+        res = shapes(a.sizes)
+    elif a.study == "precision":
         res = precision(n=a.n or 1048576, steps=a.steps or 100, plummer_a=a.plummer_a,
                         damping=a.damping, eps=a.eps)
     else:
