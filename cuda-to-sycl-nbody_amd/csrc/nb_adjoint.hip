@@ -10,13 +10,16 @@
 //   lx_k = lx'_k + c sum_{j != k} M_kj (g_j - g_k),
 //   M(r) = df/dr = K (I - 3 r r^T / d),  d = |r|^2 + eps   (symmetric, even).
 // The pair term is evaluated as K (h + t3 r) with h = g_j - g_k and
-// t3 = -3 (r . h) / d: per interaction 22 VALU ops + rsq(d^3) + rcp(d), the
-// pair quantities r, d, K recomputed in registers (no stored forward state
-// beyond the positions).  K is formed exactly as in the forward kernel.
+// t3 = -3 (r . h) / d: per interaction 22 VALU ops + one transcendental
+// (s = rsqrt(d), q = s^2 = 1/d, K = s^3), the pair quantities r, d, K
+// recomputed in registers (no stored forward state beyond the positions).
 //
 // The fused variant also accumulates F(x) = sum K r in the same pass (3 more
 // FMAs) and applies the forward step, so one launch advances the state and
-// back-propagates the cotangents at the same positions.
+// back-propagates the cotangents at the same positions.  It forms K exactly
+// as the forward kernel does (rsqrt(d^3), with q = rcp(d)), so its forward
+// outputs are bit-identical to the forward kernel's; its adjoint outputs
+// differ from the plain adjoint's by the rounding of K and q.
 //
 // Mapping: as the forward kernel (U targets per thread in VGPRs, sources in
 // LDS tiles of TILE particles, 6 SoA arrays: x, y, z, g_x, g_y, g_z; g is
@@ -60,10 +63,19 @@ struct AdjTargets {
       float d = __builtin_fmaf(rx, rx, eps);
       d = __builtin_fmaf(ry, ry, d);
       d = __builtin_fmaf(rz, rz, d);
-      // K exactly as the forward kernel forms it (rsqrt(d^3)), so the fused
-      // variant's forward outputs are bit-identical to the forward kernel's
-      float K = __builtin_amdgcn_rsqf((d * d) * d);
-      const float q = __builtin_amdgcn_rcpf(d);
+      float K, q;
+      if constexpr (FUSED) {
+        // K exactly as the forward kernel forms it (rsqrt(d^3)), so the
+        // fused variant's forward outputs are bit-identical to its
+        K = __builtin_amdgcn_rsqf((d * d) * d);
+        q = __builtin_amdgcn_rcpf(d);
+      } else {
+        // one transcendental (the pipe that bounds this loop's issue):
+        // s = d^-1/2, q = s^2 = 1/d, K = s^3
+        const float sr = __builtin_amdgcn_rsqf(d);
+        q = sr * sr;
+        K = q * sr;
+      }
       if constexpr (SKIP) K = (j == idx[k]) ? 0.0f : K;
       const float hx = gjx - gx[k];
       const float hy = gjy - gy[k];

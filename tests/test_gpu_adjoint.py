@@ -103,8 +103,9 @@ def test_adjoint_step_default_params(nb, orc, gpu, n):
 @pytest.mark.parametrize("shape", ["targets", "symmetric", "symmetric_tiles"])
 @pytest.mark.parametrize("n", [1, 2, 257, 1500, 5000])
 def test_fused_step_equals_separate(nb, gpu, n, shape):
-    """Forward part bit-equal to the forward kernel, adjoint part bit-equal
-    to the adjoint kernel (same decomposition, same summation order)."""
+    """Forward part bit-equal to the forward kernel of the same decomposition;
+    adjoint part equal to the plain adjoint kernel's up to the rounding of
+    its inverse-cube (formed from rsqrt(d) there, rsqrt(d^3) here)."""
     import torch
 
     acfg = 0 if shape == "targets" else next(
@@ -126,8 +127,12 @@ def test_fused_step_equals_separate(nb, gpu, n, shape):
     nb.adjoint_step(params, P, lxn, lvn, lx1, lv1, cfg=acfg)
     nb.fwd_adjoint_step(params, P, V2, N2, lxn, lvn, lx2, lv2, cfg=acfg)
     torch.cuda.synchronize()
-    for a, b in ((V1, V2), (N1, N2), (lx1, lx2), (lv1, lv2)):
+    for a, b in ((V1, V2), (N1, N2), (lv1, lv2)):
         assert torch.equal(a, b)
+    # the plain adjoint forms K and 1/d from one rsqrt(d), the fused one as
+    # the forward kernel does: equal up to that rounding
+    d = (lx1 - lx2).double().norm() / max(float(lx1.double().norm()), 1e-30)
+    assert d < 1e-6, float(d)
 
 
 def test_every_adjoint_shape(nb, gpu):
