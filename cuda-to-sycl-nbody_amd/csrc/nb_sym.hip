@@ -32,8 +32,10 @@
 //     sums.  Between passes the reaction sums of the four tiles live in LDS
 //     (each tile is owned by one wave per pass); after pass 3 the chunk
 //     writes them as the (GA, GB) reaction partial;
-//   * for GB == GA (first super-tile of a row) every wave sums the terms of
-//     all four tiles on its targets without reactions (self terms are +0);
+//   * for GB == GA (first super-tile of a row) each wave first meets its own
+//     tile without reactions (self terms are +0), then the tiles above it
+//     with reactions (each of the six tile pairs once), and finally
+//     subtracts the reactions its tile received;
 //   * the targets' sums over the chunk are one row partial;
 //   * a finalize kernel sums, per particle, its row partials in chunk order
 //     and then subtracts the reaction partials of its column in GA order,
@@ -419,12 +421,41 @@ __device__ __forceinline__ void super_pair(Lane<R, U> &L, const float *src,
   constexpr int P = 64 * U;
   constexpr int ST = kWaves * P;
   if (diag) {
-#pragma unroll 1
-    for (int b = 0; b < kWaves; ++b) {
-      const int j = (w + b) & (kWaves - 1);
-      tile_pass<R, U, false, MASK>(L, src, j, lane, eps, vt, gb0src + j * P,
-                                   n);
+    // Pass 0: each wave against its own tile, without reactions (self terms
+    // are +0), and zeroes that tile's reaction sums.  Pass b > 0: wave w
+    // meets tile j = w + b only if j < 4, with reactions, so each of the six
+    // tile pairs is evaluated once; tile j is visited by one wave per pass.
+    // Last, each wave subtracts the negated reactions its own tile received.
+    {
+      const int o = w * P + lane * U;
+#pragma unroll
+      for (int q = 0; q < U; ++q)
+#pragma unroll
+        for (int c = 0; c < R::ACC; ++c) sacc[c * ST + o + q] = 0.0f;
     }
+    tile_pass<R, U, false, MASK>(L, src, w, lane, eps, vt, gb0src + w * P, n);
+    __syncthreads();
+#pragma unroll 1
+    for (int b = 1; b < kWaves; ++b) {
+      const int j = w + b;
+      if (j < kWaves) {
+        const int o = j * P + lane * U;
+#pragma unroll
+        for (int q = 0; q < U; ++q)
+#pragma unroll
+          for (int c = 0; c < R::ACC; ++c) L.rs[q][c] = sacc[c * ST + o + q];
+        tile_pass<R, U, true, MASK>(L, src, j, lane, eps, vt, gb0src + j * P,
+                                    n);
+#pragma unroll
+        for (int q = 0; q < U; ++q)
+#pragma unroll
+          for (int c = 0; c < R::ACC; ++c) sacc[c * ST + o + q] = L.rs[q][c];
+      }
+      __syncthreads(); // tile j passes to the next wave
+    }
+    const int o = w * P + lane * U;
+#pragma unroll
+    for (int k = 0; k < U; ++k) R::sub_reaction(L.a[k], sacc + o + k, ST);
     return;
   }
 #pragma unroll 1
