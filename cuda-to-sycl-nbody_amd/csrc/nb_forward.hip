@@ -177,7 +177,8 @@ int forward_default_config(size_t n, int mode) {
   // scratch.
   if (mode == MODE_PLAIN && n <= kSymMaxParticles) {
     if (n > ((size_t)1 << 19)) return 23; // symmetric, 8 per lane, 2 per item
-    if (n >= ((size_t)1 << 17)) return 22; // symmetric, 8 per lane, 1 per item
+    if (n >= ((size_t)1 << 18)) return 22; // symmetric, 8 per lane, 1 per item
+    if (n >= ((size_t)1 << 17)) return 25; // symmetric, 4 per lane, 1 per item
     if (n >= 20480) return 32;             // symmetric tile pairs, 4 per lane
   }
   // Kahan, Gint/s, symmetric (4 per lane) vs target-sharded: n = 32768
