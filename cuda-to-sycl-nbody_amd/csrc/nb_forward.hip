@@ -170,8 +170,8 @@ int forward_default_config(size_t n, int mode) {
   // Measured on MI355X (disk ICs), Gint/s:
   //   symmetric, tile pairs (4 per lane): n = 25600 3750, 32768 3847,
   //     49152 4409, 65536 4669, 131072 4895;
-  //   symmetric, super-tile pairs: 8 per lane 5143 (2^17), 5761 (2^18),
-  //     5967 (2^20); 4 per lane 4627 (65536), 5341 (2^17), 5589 (2^18);
+  //   symmetric, super-tile pairs: 8 per lane 5232 (2^17), 5845 (2^18),
+  //     6042 (2^20); 4 per lane 4627 (65536), 5429 (2^17), 5611 (2^18);
   //   target-sharded: 16384 2670 (split 16), 25600 2862, 65536 3636.
   // Past 2^19 two super-tiles per item run as fast and need 25 % less
   // scratch.

@@ -283,11 +283,15 @@ def test_reference_driver_runs(nb, gpu):
     if exe.exists():
         runs.append([str(exe)])
     for cmd in runs:
-        res = subprocess.run(cmd + ["8", "2", "0.999998", "0.005", "1e-7", "2", "5", "64",
-                                    "BRANCH"], capture_output=True, text=True, timeout=120)
-        assert res.returncode == 0, res.stderr
-        lines = [l for l in res.stdout.splitlines() if l.startswith("At step")]
-        assert len(lines) == 3 and lines[-1].startswith("At step 5 kernel time is ")
+        # 2048 particles (target-sharded) and 51200 (symmetric tile pairs),
+        # frames of 2 steps (replayed hipGraphs)
+        for batches in ("8", "200"):
+            res = subprocess.run(cmd + [batches, "2", "0.999998", "0.005", "1e-7", "2", "5",
+                                        "64", "BRANCH"],
+                                 capture_output=True, text=True, timeout=120)
+            assert res.returncode == 0, res.stderr
+            lines = [l for l in res.stdout.splitlines() if l.startswith("At step")]
+            assert len(lines) == 3 and lines[-1].startswith("At step 5 kernel time is ")
 
 
 def test_staged_readback_matches_plain_reads(nb, gpu):
