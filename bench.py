@@ -199,9 +199,9 @@ def traffic_from_profile(scheme: str, n: int):
 # FLOP the hardware executes per interaction (one fused multiply-add = 2):
 #   targets:   every pair evaluated for both particles: 3 sub + 3 fma + 2 mul
 #              + rsqrt + 3 fma = 18;
-#   symmetric: each pair evaluated once for two interactions: (3 sub + 3 fma
-#              + 2 mul + rsqrt + 6 fma) / 2 = 12.
-EXECUTED_FLOP_PER_INTERACTION = {"targets": 18, "symmetric": 12}
+#   symmetric (super-tile or tile pairs): each pair evaluated once for two
+#              interactions: (3 sub + 3 fma + 2 mul + rsqrt + 6 fma) / 2 = 12.
+EXECUTED_FLOP_PER_INTERACTION = {"targets": 18, "symmetric": 12, "symmetric_tiles": 12}
 
 
 def cpu_baseline(pos, vel, n: int, seconds: float) -> dict:

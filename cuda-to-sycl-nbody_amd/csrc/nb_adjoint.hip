@@ -17,9 +17,9 @@
 // The fused variant also accumulates F(x) = sum K r in the same pass (3 more
 // FMAs) and applies the forward step, so one launch advances the state and
 // back-propagates the cotangents at the same positions.  It forms K exactly
-// as the forward kernel does (rsqrt(d^3), with q = rcp(d)), so its forward
-// outputs are bit-identical to the forward kernel's; its adjoint outputs
-// differ from the plain adjoint's by the rounding of K and q.
+// as the forward kernel does (rsqrt(d^3), and q = 1/d as K^2 d^2), so its
+// forward outputs are bit-identical to the forward kernel's; its adjoint
+// outputs differ from the plain adjoint's by the rounding of K and q.
 //
 // Mapping: as the forward kernel (U targets per thread in VGPRs, sources in
 // LDS tiles of TILE particles, 6 SoA arrays: x, y, z, g_x, g_y, g_z; g is
@@ -67,8 +67,9 @@ struct AdjTargets {
       if constexpr (FUSED) {
         // K exactly as the forward kernel forms it (rsqrt(d^3)), so the
         // fused variant's forward outputs are bit-identical to its
-        K = __builtin_amdgcn_rsqf((d * d) * d);
-        q = __builtin_amdgcn_rcpf(d);
+        const float dd = d * d;
+        K = __builtin_amdgcn_rsqf(dd * d);
+        q = (K * K) * dd; // 1/d without a second transcendental
       } else {
         // one transcendental (the pipe that bounds this loop's issue):
         // s = d^-1/2, q = s^2 = 1/d, K = s^3
@@ -296,6 +297,7 @@ const FwdConfig kAdjConfigs[] = {
     {256, 8, 2048, 1, SCHEME_SYMMETRIC},
     {256, 4, 256, 1, SCHEME_SYMMETRIC_TILES},
     {256, 2, 128, 1, SCHEME_SYMMETRIC_TILES},
+    {256, 2, 512, 1, SCHEME_SYMMETRIC}, {256, 2, 512, 2, SCHEME_SYMMETRIC},
 };
 constexpr int kNumAdj = sizeof(kAdjConfigs) / sizeof(kAdjConfigs[0]);
 
@@ -312,18 +314,18 @@ int adjoint_config_count() { return kNumAdj; }
 const FwdConfig &adjoint_config(int cfg) { return kAdjConfigs[cfg]; }
 
 int adjoint_default_config(size_t n, bool skip) {
-  // Measured on MI355X (disk ICs), adjoint / fused ms:
-  //   tile pairs, 2 per lane: n = 2048 0.030 / 0.037, 4096 0.033 / 0.039,
-  //     16384 0.165 / 0.189, 32768 0.486 / 0.568;
-  //   tile pairs, 4 per lane: 2048 0.066 / 0.079, 16384 0.197 / 0.236,
-  //     65536 1.72 / 2.17;
-  //   super-tile pairs: 65536 1.84 / 2.20, 262144 25.5 / 31.3;
-  //   best target-sharded: 2048 0.066 / 0.071, 16384 0.427 / 0.468,
-  //     262144 38.7 / 43.5.
+  // Measured on MI355X (disk ICs), adjoint / fused us.  Tile pairs, 2 per
+  // lane: n = 4096 28.7 / 35.7, 16384 126.2 / 144.4, 32768 379.9 / 449.9.
+  // Super-tile pairs, 2 per lane, 1 / 2 per item: 32768 354.7 / 471.7 vs
+  // 403.2 / 542.8, 65536 1219 / 1644 (tile pairs 1447 / 1771), 131072
+  // 4780 / 6511 vs 4779 / 6478, 262144 19280 / 25892 vs 19168 / 25601
+  // (4 per lane 23660 / 31135, 8 per lane 24283 / 34040).  Best
+  // target-sharded: 8192 65.8 / 74.2, 262144 38.7 / 43.5 ms.
   if (!skip && n <= kSymMaxParticles) {
-    if (n >= ((size_t)1 << 17)) return 14; // super-tile pairs, 4 per lane
-    if (n >= 49152) return 17;             // tile pairs, 4 per lane
-    if (n >= 1024) return 18;              // tile pairs, 2 per lane
+    if (n > kSymSmallMaxParticles) return 14; // super-tile pairs, 4 per lane
+    if (n >= 98304) return 20; // super-tile pairs, 2 per lane, 2 per item
+    if (n >= 32768) return 19; // super-tile pairs, 2 per lane, 1 per item
+    if (n >= 1024) return 18;  // tile pairs, 2 per lane
   }
   if (n >= (size_t)1 << 17) return 8; // 256 x 1, split 2
   if (n < 4096) return 8;

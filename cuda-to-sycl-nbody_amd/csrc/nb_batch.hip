@@ -130,6 +130,7 @@ const BatchShape kBatchShapes[] = {
     // symmetric (nb_sym.hip): 4 waves, particles per lane, super-tile
     {256, 4, 1024, nullptr, SCHEME_SYMMETRIC},
     {256, 8, 2048, nullptr, SCHEME_SYMMETRIC},
+    {256, 2, 512, nullptr, SCHEME_SYMMETRIC},
 };
 constexpr int kNumBatchShapes = sizeof(kBatchShapes) / sizeof(kBatchShapes[0]);
 

@@ -150,6 +150,7 @@ const FwdConfig kConfigs[] = {
     {128, 1, 512, 16},
     {256, 4, 256, 1, SCHEME_SYMMETRIC_TILES},
     {256, 2, 128, 1, SCHEME_SYMMETRIC_TILES},
+    {256, 2, 512, 1, SCHEME_SYMMETRIC}, {256, 2, 512, 2, SCHEME_SYMMETRIC},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
@@ -167,19 +168,21 @@ int forward_config_count() { return kNumConfigs; }
 const FwdConfig &forward_config(int cfg) { return kConfigs[cfg]; }
 
 int forward_default_config(size_t n, int mode) {
-  // Measured on MI355X (disk ICs), Gint/s:
-  //   symmetric, tile pairs (4 per lane): n = 25600 3750, 32768 3847,
-  //     49152 4409, 65536 4669, 131072 4895;
-  //   symmetric, super-tile pairs: 8 per lane 5232 (2^17), 5845 (2^18),
-  //     6042 (2^20); 4 per lane 4627 (65536), 5429 (2^17), 5611 (2^18);
-  //   target-sharded: 16384 2670 (split 16), 25600 2862, 65536 3636.
-  // Past 2^19 two super-tiles per item run as fast and need 25 % less
-  // scratch.
+  // Measured on MI355X (disk ICs), us per step.  Tile pairs, 2 per lane:
+  // n = 4096 22.6 (target-sharded 128 x 1 split 8: 29), 8192 38.1 (38.7),
+  // 12800 57.5 (82.5), 32768 238.4 (323.6; 4 per lane 266.3).  Super-tile
+  // pairs, 2 per lane, 1 / 2 per item: 49152 481.8 / 532.7 (tile pairs
+  // 499.0), 65536 810.7 / 842.7 (867.9), 98304 1764 / 1780 (4 per lane
+  // 1860, 8 per lane 1915), 131072 3118 / 3104 (4 per lane 3176, 8 per lane
+  // 3299), 262144 12236 / 12133 (8 per lane 11804).  Past 2^19 two
+  // super-tiles of 8 per lane per item run as fast as one and need 25 %
+  // less scratch (6042 Gint/s at 2^20).
   if (mode == MODE_PLAIN && n <= kSymMaxParticles) {
     if (n > ((size_t)1 << 19)) return 23; // symmetric, 8 per lane, 2 per item
     if (n >= ((size_t)1 << 18)) return 22; // symmetric, 8 per lane, 1 per item
-    if (n >= ((size_t)1 << 17)) return 25; // symmetric, 4 per lane, 1 per item
-    if (n >= 20480) return 32;             // symmetric tile pairs, 4 per lane
+    if (n >= 98304) return 35;             // symmetric, 2 per lane, 2 per item
+    if (n >= 49152) return 34;             // symmetric, 2 per lane, 1 per item
+    if (n >= 4096) return 33;              // symmetric tile pairs, 2 per lane
   }
   // Kahan, Gint/s, symmetric (4 per lane) vs target-sharded: n = 32768
   // tile pairs 1905 vs 1880; 65536 2152 vs 2123; 262144 super-tile pairs

@@ -71,6 +71,15 @@ int forward_default_config(size_t n, int mode);
 constexpr size_t kSymMaxParticles = (size_t)1 << 20;
 // Largest particle count of the tile-pair form (0.8 GB of partials at 2^17).
 constexpr size_t kSymTileMaxParticles = (size_t)1 << 17;
+// Largest particle count of the super-tile shapes with 2 particles per lane
+// (1.6 GB of partials at 2^18).
+constexpr size_t kSymSmallMaxParticles = (size_t)1 << 18;
+
+// Largest particle count of a symmetric launch shape.
+constexpr size_t sym_max_particles(int unroll, bool tiles) {
+  return tiles ? kSymTileMaxParticles
+               : unroll <= 2 ? kSymSmallMaxParticles : kSymMaxParticles;
+}
 
 // Symmetric scheme (nb_sym.hip; vanishing self term only): scratch bytes
 // for `acc` sums per particle (3 forward and adjoint, 6 fused adjoint) and

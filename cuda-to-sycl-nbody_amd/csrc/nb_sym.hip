@@ -54,8 +54,8 @@
 //
 // Per pair term (two interactions): forward 14 VALU + v_rsq_f32 + 3/U DPP
 // moves; adjoint 25 VALU + v_rsq_f32 (K and 1/d from rsqrt(d)) + 3/U DPP;
-// fused adjoint 31 VALU + v_rsq_f32 + v_rcp_f32 (K as the forward rule forms
-// it) + 6/U DPP.  The target-sharded kernels spend 11 (forward) and 22 (adjoint)
+// fused adjoint 33 VALU + v_rsq_f32 (K as the forward rule forms it, 1/d =
+// K^2 d^2) + 6/U DPP.  The target-sharded kernels spend 11 (forward) and 22 (adjoint)
 // VALU per interaction.
 //
 // Build with -ffp-contract=off (explicit fmaf only).
@@ -117,18 +117,31 @@ struct SymArgs {
 
 // Finalize arithmetic of the plain rules: row partials are added, the
 // negated reaction partials subtracted, in the order the caller visits them.
+// fetch reads one partial ([ACC] floats, stride apart); add_v / sub_v fold it
+// into the running sums.
 template <int ACC>
 struct PlainPartials {
-  static __device__ __forceinline__ void add_row(float (&f)[ACC],
-                                                 const float *p, int stride) {
+  static __device__ __forceinline__ void fetch(float (&v)[ACC], const float *p,
+                                               int stride) {
 #pragma unroll
-    for (int e = 0; e < ACC; ++e) f[e] = f[e] + p[e * stride];
+    for (int e = 0; e < ACC; ++e) v[e] = p[e * stride];
+  }
+  static __device__ __forceinline__ void add_v(float (&f)[ACC],
+                                               const float (&v)[ACC]) {
+#pragma unroll
+    for (int e = 0; e < ACC; ++e) f[e] = f[e] + v[e];
+  }
+  static __device__ __forceinline__ void sub_v(float (&f)[ACC],
+                                               const float (&v)[ACC]) {
+#pragma unroll
+    for (int e = 0; e < ACC; ++e) f[e] = f[e] - v[e];
   }
   static __device__ __forceinline__ void sub_reaction(float (&f)[ACC],
                                                       const float *p,
                                                       int stride) {
-#pragma unroll
-    for (int e = 0; e < ACC; ++e) f[e] = f[e] - p[e * stride];
+    float v[ACC];
+    fetch(v, p, stride);
+    sub_v(f, v);
   }
 };
 
@@ -224,18 +237,27 @@ struct KahanForceRule {
     f[3 + e] = (t - f[e]) - y;
     f[e] = t;
   }
-  static __device__ __forceinline__ void add_row(float (&f)[ACC],
-                                                 const float *p, int stride) {
+  static __device__ __forceinline__ void fetch(float (&v)[ACC], const float *p,
+                                               int stride) {
 #pragma unroll
-    for (int e = 0; e < 3; ++e)
-      kahan_value(f, e, p[e * stride] - p[(3 + e) * stride]);
+    for (int e = 0; e < ACC; ++e) v[e] = p[e * stride];
+  }
+  static __device__ __forceinline__ void add_v(float (&f)[ACC],
+                                               const float (&v)[ACC]) {
+#pragma unroll
+    for (int e = 0; e < 3; ++e) kahan_value(f, e, v[e] - v[3 + e]);
+  }
+  static __device__ __forceinline__ void sub_v(float (&f)[ACC],
+                                               const float (&v)[ACC]) {
+#pragma unroll
+    for (int e = 0; e < 3; ++e) kahan_value(f, e, v[3 + e] - v[e]);
   }
   static __device__ __forceinline__ void sub_reaction(float (&f)[ACC],
                                                       const float *p,
                                                       int stride) {
-#pragma unroll
-    for (int e = 0; e < 3; ++e)
-      kahan_value(f, e, p[(3 + e) * stride] - p[e * stride]);
+    float v[ACC];
+    fetch(v, p, stride);
+    sub_v(f, v);
   }
 
   static __device__ __forceinline__ void finish(const SymArgs &a, int i,
@@ -281,8 +303,11 @@ struct AdjointRule : PlainPartials<FUSED ? 6 : 3> {
     d = __builtin_fmaf(rz, rz, d);
     float K, q;
     if constexpr (FUSED) { // K exactly as the forward kernels form it
-      K = __builtin_amdgcn_rsqf((d * d) * d);
-      q = __builtin_amdgcn_rcpf(d);
+      // 1/d = K^2 d^2: two multiplies instead of v_rcp_f32 (2 % faster at
+      // N = 262144, 8 % with tile pairs at N = 16384)
+      const float dd = d * d;
+      K = __builtin_amdgcn_rsqf(dd * d);
+      q = (K * K) * dd;
     } else { // one transcendental (8 % faster): s = d^-1/2, q = s^2, K = s^3
       const float sr = __builtin_amdgcn_rsqf(d);
       q = sr * sr;
@@ -563,6 +588,42 @@ __device__ __forceinline__ void store_row(const Lane<R, U> &T, float *fp,
     for (int e = 0; e < R::ACC; ++e) q[e * ST + k] = T.a[k][e];
 }
 
+// f += partial(0), partial(1), ... in that order (SUB: f -= ...), with
+// kInFlight partials loaded ahead of the running sum: the loads do not
+// depend on it, so a finalize thread waits about once per kInFlight
+// partials instead of once per partial.  The order, and so the result, is
+// that of the plain loop.
+constexpr int kInFlight = 8;
+
+template <class R, bool SUB, class At>
+__device__ __forceinline__ void sum_partials(float (&f)[R::ACC], int count,
+                                             At at, int stride) {
+  int k = 0;
+  for (; k + kInFlight <= count; k += kInFlight) {
+    float v[kInFlight][R::ACC];
+#pragma unroll
+    for (int u = 0; u < kInFlight; ++u) R::fetch(v[u], at(k + u), stride);
+#pragma unroll
+    for (int u = 0; u < kInFlight; ++u) {
+      if constexpr (SUB) R::sub_v(f, v[u]);
+      else R::add_v(f, v[u]);
+    }
+  }
+  if (k < count) { // the last count - k < kInFlight partials, loaded together
+    float v[kInFlight][R::ACC];
+#pragma unroll
+    for (int u = 0; u < kInFlight; ++u)
+      if (k + u < count) R::fetch(v[u], at(k + u), stride);
+#pragma unroll
+    for (int u = 0; u < kInFlight; ++u) {
+      if (k + u < count) {
+        if constexpr (SUB) R::sub_v(f, v[u]);
+        else R::add_v(f, v[u]);
+      }
+    }
+  }
+}
+
 template <class R, int U>
 __global__ __launch_bounds__(256) void sym_kernel(SymArgs a) {
   constexpr int P = 64 * U;
@@ -612,11 +673,16 @@ __global__ __launch_bounds__(256) void sym_finalize(SymArgs a) {
   float f[R::ACC];
 #pragma unroll
   for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
-  for (int c = 0; c < nc; ++c)
-    R::add_row(f, a.fpart + (size_t)(first + c) * (R::ACC * ST) + o, ST);
-  for (int ga = 0; ga < g; ++ga) // negated reaction sums
-    R::sub_reaction(
-        f, a.spart + (size_t)pair_slot(ga, g, ts) * (R::ACC * ST) + o, ST);
+  sum_partials<R, false>(
+      f, nc,
+      [&](int c) { return a.fpart + (size_t)(first + c) * (R::ACC * ST) + o; },
+      ST);
+  sum_partials<R, true>( // negated reaction sums
+      f, g,
+      [&](int ga) {
+        return a.spart + (size_t)pair_slot(ga, g, ts) * (R::ACC * ST) + o;
+      },
+      ST);
   R::finish(a, i, f);
 }
 
@@ -718,14 +784,19 @@ __global__ __launch_bounds__(256) void sym_tile_finalize(SymArgs a) {
   float f[R::ACC];
 #pragma unroll
   for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
-  for (int b = g; b < T; ++b)
-    R::add_row(f, a.fpart + (size_t)tri_slot(g, b, T) * (2 * R::ACC * P) + o,
-               P);
-  for (int c = 0; c < g; ++c)
-    R::sub_reaction(f,
-                    a.fpart + (size_t)tri_slot(c, g, T) * (2 * R::ACC * P) +
-                        R::ACC * P + o,
-                    P);
+  sum_partials<R, false>(
+      f, T - g,
+      [&](int k) {
+        return a.fpart + (size_t)tri_slot(g, g + k, T) * (2 * R::ACC * P) + o;
+      },
+      P);
+  sum_partials<R, true>(
+      f, g,
+      [&](int c) {
+        return a.fpart + (size_t)tri_slot(c, g, T) * (2 * R::ACC * P) +
+               R::ACC * P + o;
+      },
+      P);
   R::finish(a, i, f);
 }
 
@@ -839,14 +910,20 @@ __global__ __launch_bounds__(256) void sym_batch_finalize(SymBatchArgs b) {
   float f[R::ACC];
 #pragma unroll
   for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
-  for (int gb = g; gb < a.ts; ++gb)
-    R::add_row(f, base + (size_t)tri_slot(g, gb, a.ts) * (2 * R::ACC * ST),
-               ST);
-  for (int ga = 0; ga < g; ++ga) // negated reaction sums
-    R::sub_reaction(f,
-                    base + (size_t)tri_slot(ga, g, a.ts) * (2 * R::ACC * ST) +
-                        R::ACC * ST,
-                    ST);
+  const int ts = a.ts;
+  sum_partials<R, false>(
+      f, ts - g,
+      [&](int k) {
+        return base + (size_t)tri_slot(g, g + k, ts) * (2 * R::ACC * ST);
+      },
+      ST);
+  sum_partials<R, true>( // negated reaction sums
+      f, g,
+      [&](int ga) {
+        return base + (size_t)tri_slot(ga, g, ts) * (2 * R::ACC * ST) +
+               R::ACC * ST;
+      },
+      ST);
   R::finish(a, li, f);
 }
 
@@ -860,6 +937,7 @@ struct SymShape {
 template <class R>
 const SymShape *find_shape(int unroll, bool tiles) {
   static const SymShape shapes[] = {
+      {2, sym_kernel<R, 2>, sym_finalize<R, 2>},
       {4, sym_kernel<R, 4>, sym_finalize<R, 4>},
       {8, sym_kernel<R, 8>, sym_finalize<R, 8>},
   };
@@ -889,10 +967,12 @@ const SymShape *find_kahan_shape(int unroll, bool tiles) {
 
 int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
                float *scratch, int acc, bool tiles, hipStream_t stream) {
-  if ((size_t)n > (tiles ? kSymTileMaxParticles : kSymMaxParticles))
-    return set_error(NB_ERR_UNSUPPORTED,
-                     tiles ? "symmetric tile-pair shapes take at most 2^17 particles"
-                           : "symmetric launch shapes take at most 2^20 particles");
+  if ((size_t)n > sym_max_particles(unroll, tiles))
+    return set_error(
+        NB_ERR_UNSUPPORTED,
+        tiles         ? "symmetric tile-pair shapes take at most 2^17 particles"
+        : unroll <= 2 ? "symmetric shapes with 2 per lane take at most 2^18 particles"
+                      : "symmetric launch shapes take at most 2^20 particles");
   if (!sh || chunk < 1)
     return set_error(NB_ERR_UNSUPPORTED, "no symmetric kernel for this shape");
   if (!scratch)
@@ -933,12 +1013,12 @@ int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
 size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc,
                          bool tiles) {
   if (tiles) {
-    if (n == 0 || n > kSymTileMaxParticles) return 0;
+    if (n == 0 || n > sym_max_particles(unroll, tiles)) return 0;
     const long long P = 64LL * unroll;
     const long long T = ((long long)n + P - 1) / P;
     return sizeof(float) * (size_t)(T * (T + 1) / 2) * 2 * acc * (size_t)P;
   }
-  if (n == 0 || n > kSymMaxParticles || chunk < 1) return 0;
+  if (n == 0 || n > sym_max_particles(unroll, tiles) || chunk < 1) return 0;
   const long long st = (long long)kWaves * 64 * unroll;
   const long long ts = ((long long)n + st - 1) / st;
   const long long slots = items_upto(ts, chunk) + ts * (ts - 1) / 2;
@@ -948,7 +1028,7 @@ size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc,
 int sym_batch_layout(int unroll, const int *sys_off, int num_systems,
                      std::vector<int4> &items, std::vector<int> &sys_slot,
                      size_t &part_bytes) {
-  if (unroll != 4 && unroll != 8)
+  if (unroll != 2 && unroll != 4 && unroll != 8)
     return set_error(NB_ERR_UNSUPPORTED, "no symmetric batch kernel for shape");
   const int ST = kWaves * 64 * unroll;
   std::vector<int4> off_diag, diag;
@@ -985,13 +1065,20 @@ template <int U> const void *batch_finalize_fn() {
 } // namespace
 
 const void *sym_batch_kernel_fn(int unroll) {
-  return unroll == 4 ? batch_pairs_fn<4>() : unroll == 8 ? batch_pairs_fn<8>() : nullptr;
+  switch (unroll) {
+  case 2: return batch_pairs_fn<2>();
+  case 4: return batch_pairs_fn<4>();
+  case 8: return batch_pairs_fn<8>();
+  default: return nullptr;
+  }
 }
 
 int launch_sym_batch(int unroll, const SymBatchLaunch &L, hipStream_t stream) {
   const void *pairs = sym_batch_kernel_fn(unroll);
-  const void *fin = unroll == 4 ? batch_finalize_fn<4>()
-                                : unroll == 8 ? batch_finalize_fn<8>() : nullptr;
+  const void *fin = unroll == 2   ? batch_finalize_fn<2>()
+                    : unroll == 4 ? batch_finalize_fn<4>()
+                    : unroll == 8 ? batch_finalize_fn<8>()
+                                  : nullptr;
   if (!pairs || !fin)
     return set_error(NB_ERR_UNSUPPORTED, "no symmetric batch kernel for shape");
   if (L.num_items == 0) return NB_OK;

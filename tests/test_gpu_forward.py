@@ -424,7 +424,8 @@ def test_symmetric_scheme_rejects_other_modes(nb, gpu):
     sim.set_accumulation(nb.NB_ACC_PLAIN)
     assert nb.forward_config_info(sim.config)["scheme"] != "targets"
     sim.close()
-    # beyond 2^20 particles (2^17 for tile pairs): refused before any launch
+    # beyond 2^20 particles (2^17 for tile pairs, 2^18 with 2 per lane): refused
+    # before any launch
     m = (1 << 20) + 1
     big = [torch.zeros(m, device=gpu) for _ in range(9)]
     with pytest.raises(nb.NbError, match="2\\^20"):
@@ -436,6 +437,13 @@ def test_symmetric_scheme_rejects_other_modes(nb, gpu):
     with pytest.raises(nb.NbError, match="2\\^17"):
         nb.forward_step(nb.default_params(num_particles=m), [t[:m] for t in big[:3]],
                         [t[:m] for t in big[3:6]], [t[:m] for t in big[6:]], cfg=tiles)
+    small = next(c for c in sym_configs(nb)  # super-tiles of 2 per lane: 2^18
+                 if nb.forward_config_info(c)["scheme"] == "symmetric"
+                 and nb.forward_config_info(c)["unroll"] == 2)
+    m = (1 << 18) + 1
+    with pytest.raises(nb.NbError, match="2\\^18"):
+        nb.forward_step(nb.default_params(num_particles=m), [t[:m] for t in big[:3]],
+                        [t[:m] for t in big[3:6]], [t[:m] for t in big[6:]], cfg=small)
 
 
 @pytest.mark.parametrize("n", [3000, 25600])
