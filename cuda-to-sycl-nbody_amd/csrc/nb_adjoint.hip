@@ -315,8 +315,9 @@ const FwdConfig &adjoint_config(int cfg) { return kAdjConfigs[cfg]; }
 
 int adjoint_default_config(size_t n, bool skip) {
   // Measured on MI355X (disk ICs), adjoint / fused us.  Tile pairs, 2 per
-  // lane: n = 4096 28.7 / 35.7, 16384 126.2 / 144.4, 32768 379.9 / 449.9.
-  // Super-tile pairs, 2 per lane, 1 / 2 per item: 32768 354.7 / 471.7 vs
+  // lane: n = 4096 28.7 / 35.7, 16384 124.5 / 146.3 (super-tile pairs, 2 per
+  // lane: 141.7 / 181.0), 25600 245.8 / 289.7 (217.0 / 282.2), 32768 379.9 /
+  // 449.9.  Super-tile pairs, 2 per lane, 1 / 2 per item: 32768 354.7 / 471.7 vs
   // 403.2 / 542.8, 65536 1219 / 1644 (tile pairs 1447 / 1771), 131072
   // 4780 / 6511 vs 4779 / 6478, 262144 19280 / 25892 vs 19168 / 25601
   // (4 per lane 23660 / 31135, 8 per lane 24283 / 34040).  Best
@@ -324,7 +325,7 @@ int adjoint_default_config(size_t n, bool skip) {
   if (!skip && n <= kSymMaxParticles) {
     if (n > kSymSmallMaxParticles) return 14; // super-tile pairs, 4 per lane
     if (n >= 98304) return 20; // super-tile pairs, 2 per lane, 2 per item
-    if (n >= 32768) return 19; // super-tile pairs, 2 per lane, 1 per item
+    if (n >= 20480) return 19; // super-tile pairs, 2 per lane, 1 per item
     if (n >= 1024) return 18;  // tile pairs, 2 per lane
   }
   if (n >= (size_t)1 << 17) return 8; // 256 x 1, split 2

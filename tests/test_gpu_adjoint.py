@@ -44,41 +44,43 @@ def matching_forward_config(nb, acfg):
     raise AssertionError(f"no forward shape like {a}")
 
 
-@pytest.mark.parametrize("cfg", ["default", "sym0", "sym1", "sym2", "sym3"])
-def test_adjoint_pair_sum_vs_fp64(nb, orc, gpu, cfg):
+@pytest.mark.parametrize("which", ["default", "symmetric"])
+def test_adjoint_pair_sum_vs_fp64(nb, orc, gpu, which):
     """With dt = G = 1 and lam_x' = 0 the kernel's lam_x is the pair sum
     J^T g itself; compare it per particle with the fp64 sum (3000 particles:
-    a partial last tile for every shape)."""
+    a partial last tile for every shape), for the library's shape and for
+    every symmetric shape."""
     import torch
 
-    cfg = -1 if cfg == "default" else sym_adjoint_configs(nb)[int(cfg[3:])]
-
+    cfgs = [-1] if which == "default" else sym_adjoint_configs(nb)
     n = 3000
     pos, _ = nb.plummer_ic(n, seed=21, g_mass=1.0)
     rng = np.random.default_rng(2)
     g = rng.normal(size=(3, n)).astype(np.float32)
     params = nb.default_params(num_particles=n, dt=1.0, G=1.0, damping=0.5)
+    p = orc.params(dt=1.0, G=1.0, damping=0.5)
+    ref_lx, ref_lv = orc.adjoint_step_f64(p, pos, np.zeros((3, n)), g.astype(np.float64))
+    ref = np.stack(ref_lx)
     P = to_dev(torch, pos, gpu)
     lxn = torch.zeros(3, n, device=gpu)
     lvn = torch.from_numpy(g).to(gpu)
-    lx = torch.empty(3, n, device=gpu)
-    lv = torch.empty(3, n, device=gpu)
-    nb.adjoint_step(params, P, lxn, lvn, lx, lv, cfg=cfg)
-    torch.cuda.synchronize()
-    p = orc.params(dt=1.0, G=1.0, damping=0.5)
-    ref_lx, ref_lv = orc.adjoint_step_f64(p, pos, np.zeros((3, n)), g.astype(np.float64))
-    got = lx.cpu().numpy().astype(np.float64)
-    ref = np.stack(ref_lx)
-    e = np.abs(got - ref).max(0) / np.abs(ref).max()
-    print(f"J^T g vs fp64: max {e.max():.2e} normwise {rel(got, ref):.2e}")
-    assert rel(got, ref) < 1e-5 and e.max() < 1e-4
-    assert np.array_equal(lv.cpu().numpy(), np.float32(0.5) * g)
+    for cfg in cfgs:
+        lx = torch.empty(3, n, device=gpu)
+        lv = torch.empty(3, n, device=gpu)
+        nb.adjoint_step(params, P, lxn, lvn, lx, lv, cfg=cfg)
+        torch.cuda.synchronize()
+        got = lx.cpu().numpy().astype(np.float64)
+        e = np.abs(got - ref).max(0) / np.abs(ref).max()
+        info = nb.adjoint_config_info(cfg) if cfg >= 0 else "default"
+        print(f"J^T g vs fp64 ({info}): max {e.max():.2e} normwise {rel(got, ref):.2e}")
+        assert rel(got, ref) < 1e-5 and e.max() < 1e-4, info
+        assert np.array_equal(lv.cpu().numpy(), np.float32(0.5) * g), info
 
 
 @pytest.mark.parametrize("n", [4096, 33000])
 def test_adjoint_step_default_params(nb, orc, gpu, n):
-    """SimParam defaults, the library's shape (target-sharded at 4096,
-    symmetric at 33000), against the fp64 adjoint."""
+    """SimParam defaults, the library's shape (tile pairs at 4096, super-tile
+    pairs at 33000), against the fp64 adjoint."""
     import torch
 
     pos, _ = nb.disk_ic(n)
@@ -105,13 +107,18 @@ def test_adjoint_step_default_params(nb, orc, gpu, n):
 def test_fused_step_equals_separate(nb, gpu, n, shape):
     """Forward part bit-equal to the forward kernel of the same decomposition;
     adjoint part equal to the plain adjoint kernel's up to the rounding of
-    its inverse-cube (formed from rsqrt(d) there, rsqrt(d^3) here)."""
+    its inverse-cube and 1/d (formed from rsqrt(d) there, from rsqrt(d^3)
+    here); every shape of each scheme."""
+    acfgs = [0] if shape == "targets" else [
+        c for c in sym_adjoint_configs(nb) if nb.adjoint_config_info(c)["scheme"] == shape]
+    for acfg in acfgs:
+        fused_equals_separate(nb, gpu, n, acfg)
+
+
+def fused_equals_separate(nb, gpu, n, acfg):
     import torch
 
-    acfg = 0 if shape == "targets" else next(
-        c for c in sym_adjoint_configs(nb) if nb.adjoint_config_info(c)["scheme"] == shape)
     fcfg = matching_forward_config(nb, acfg)
-
     pos, vel = nb.plummer_ic(n, seed=n, g_mass=1.0)
     params = nb.default_params(num_particles=n, G=1.0 / n)
     rng = np.random.default_rng(n)
@@ -128,11 +135,11 @@ def test_fused_step_equals_separate(nb, gpu, n, shape):
     nb.fwd_adjoint_step(params, P, V2, N2, lxn, lvn, lx2, lv2, cfg=acfg)
     torch.cuda.synchronize()
     for a, b in ((V1, V2), (N1, N2), (lv1, lv2)):
-        assert torch.equal(a, b)
+        assert torch.equal(a, b), acfg
     # the plain adjoint forms K and 1/d from one rsqrt(d), the fused one as
     # the forward kernel does: equal up to that rounding
     d = (lx1 - lx2).double().norm() / max(float(lx1.double().norm()), 1e-30)
-    assert d < 1e-6, float(d)
+    assert d < 1e-6, (acfg, float(d))
 
 
 def test_every_adjoint_shape(nb, gpu):
