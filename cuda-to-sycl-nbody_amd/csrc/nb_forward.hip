@@ -184,16 +184,21 @@ int forward_default_config(size_t n, int mode) {
     if (n >= 49152) return 34;             // symmetric, 2 per lane, 1 per item
     if (n >= 4096) return 33;              // symmetric tile pairs, 2 per lane
   }
-  // Kahan, Gint/s, symmetric (4 per lane) vs target-sharded: n = 32768
-  // tile pairs 1905 vs 1880; 65536 2152 vs 2123; 262144 super-tile pairs
-  // 2445 vs 2379; 2^20 2469 vs 2474.  The compensated sums' dependency
-  // chains, not the pair count, bound this mode; above 2^18 the symmetric
-  // partials ((sum, compensation) pairs) would need gigabytes of scratch.
+  // Kahan, Gint/s: tile pairs, 2 per lane: n = 16384 1961 (best
+  // target-sharded 1664), 32768 2408 (1847); super-tile pairs, 2 per lane,
+  // 1 / 2 per item: 65536 2774 / 2674, 131072 2873 / 2867, 262144 2916 /
+  // 2929 (4 per lane 2411, target-sharded 2373).  The compensated sums'
+  // dependency chains bound this mode, so the shapes with the most waves
+  // win; below 12288 the target-sharded ones (8192: 1229 vs 1128), above
+  // 2^18 the symmetric partials ((sum, compensation) pairs) would need
+  // gigabytes of scratch.
   if (mode == MODE_KAHAN) {
     if (n >= ((size_t)1 << 17) && n <= ((size_t)1 << 18))
-      return 25;               // symmetric, 4 per lane, 1 per item
-    if (n >= 20480 && n < ((size_t)1 << 17))
-      return 32;               // symmetric tile pairs, 4 per lane
+      return 35;               // symmetric, 2 per lane, 2 per item
+    if (n >= 49152 && n < ((size_t)1 << 17))
+      return 34;               // symmetric, 2 per lane, 1 per item
+    if (n >= 12288 && n < 49152)
+      return 33;               // symmetric tile pairs, 2 per lane
   }
   // Target-sharded, Gint/s: n = 4096: 128 x 1 split 8 629 (256 x 1 split 8
   // 480); 8192: 256 x 1 split 8 1746 (split 16 1629); 16384 .. 32768:
@@ -226,9 +231,9 @@ int launch_forward(const StepConsts &c, int mode, int cfg, const float *px,
       return set_error(NB_ERR_UNSUPPORTED,
                        "symmetric launch shapes take BRANCH with dist_eps^3 "
                        "a normal float");
-    if (mode == MODE_KAHAN && fc.unroll != 4)
+    if (mode == MODE_KAHAN && fc.unroll > 4)
       return set_error(NB_ERR_UNSUPPORTED,
-                       "symmetric Kahan shapes have 4 particles per lane");
+                       "symmetric Kahan shapes have 2 or 4 particles per lane");
     return launch_forward_sym(c, fc.unroll, fc.split,
                               fc.scheme == SCHEME_SYMMETRIC_TILES,
                               mode == MODE_KAHAN, px, py, pz, vx, vy, vz, nx,

@@ -33,8 +33,8 @@
 //     (each tile is owned by one wave per pass); after pass 3 the chunk
 //     writes them as the (GA, GB) reaction partial;
 //   * for GB == GA (first super-tile of a row) each wave meets its own tile
-//     without reactions (self terms are +0) and then, with 4 per lane, the
-//     tiles above it with reactions (each of the six tile pairs once),
+//     without reactions (self terms are +0) and then, with 2 or 4 per lane,
+//     the tiles above it with reactions (each of the six tile pairs once),
 //     subtracting at the end the reactions its tile received; with 8 per
 //     lane it meets the other three tiles without reactions as well;
 //   * the targets' sums over the chunk are one row partial;
@@ -55,8 +55,8 @@
 // Per pair term (two interactions): forward 14 VALU + v_rsq_f32 + 3/U DPP
 // moves; adjoint 25 VALU + v_rsq_f32 (K and 1/d from rsqrt(d)) + 3/U DPP;
 // fused adjoint 33 VALU + v_rsq_f32 (K as the forward rule forms it, 1/d =
-// K^2 d^2) + 6/U DPP.  The target-sharded kernels spend 11 (forward) and 22 (adjoint)
-// VALU per interaction.
+// K^2 d^2) + 6/U DPP.  The target-sharded kernels spend 11 (forward) and 22
+// (adjoint) VALU per interaction.
 //
 // Build with -ffp-contract=off (explicit fmaf only).
 
@@ -448,8 +448,9 @@ __device__ __forceinline__ void super_pair(Lane<R, U> &L, const float *src,
   constexpr int ST = kWaves * P;
   // Diagonal super-tile, two measured strategies: with 8 per lane every wave
   // meets all four tiles without reactions (16 tile passes, all waves equally
-  // long: 11.77 ms at N = 262144 against 12.01 ms the other way); with 4 per
-  // lane the six tile pairs with reactions (13.13 -> 12.30 ms).
+  // long: 11.77 ms at N = 262144 against 12.01 ms the other way); with 2 or
+  // 4 per lane the six tile pairs with reactions (4 per lane: 13.13 -> 12.30
+  // ms).
   constexpr bool kDiagReactions = U <= 4;
   if (diag && !kDiagReactions) {
 #pragma unroll 1
@@ -955,14 +956,22 @@ const SymShape *find_shape(int unroll, bool tiles) {
   return nullptr;
 }
 
-// Kahan shapes: 4 per lane only (the compensations double the sums).
+// Kahan shapes: 2 or 4 per lane (the compensations double the sums).
 const SymShape *find_kahan_shape(int unroll, bool tiles) {
-  static const SymShape super4 = {4, sym_kernel<KahanForceRule, 4>,
-                                  sym_finalize<KahanForceRule, 4>};
-  static const SymShape tiles4 = {4, sym_tile_kernel<KahanForceRule, 4>,
-                                  sym_tile_finalize<KahanForceRule, 4>};
-  if (unroll != 4) return nullptr;
-  return tiles ? &tiles4 : &super4;
+  using R = KahanForceRule;
+  static const SymShape shapes[] = {
+      {2, sym_kernel<R, 2>, sym_finalize<R, 2>},
+      {4, sym_kernel<R, 4>, sym_finalize<R, 4>},
+  };
+  static const SymShape tile_shapes[] = {
+      {2, sym_tile_kernel<R, 2>, sym_tile_finalize<R, 2>},
+      {4, sym_tile_kernel<R, 4>, sym_tile_finalize<R, 4>},
+  };
+  for (int k = 0; k < 2; ++k) {
+    const SymShape &s = tiles ? tile_shapes[k] : shapes[k];
+    if (s.unroll == unroll) return &s;
+  }
+  return nullptr;
 }
 
 int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,

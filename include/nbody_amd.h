@@ -49,8 +49,8 @@ extern "C" {
 /* Force-sum accumulation mode (extension; BRANCH only). */
 #define NB_ACC_PLAIN 0 /* one fp32 FMA per term (default)                      */
 #define NB_ACC_KAHAN 1 /* Kahan-compensated fp32 sum of the per-particle force
-                          (target-sharded shapes, and symmetric shapes with 4
-                          particles per lane)                                  */
+                          (target-sharded shapes, and symmetric shapes with 2
+                          or 4 particles per lane)                             */
 
 /* SimParam, field for field (src/sim_param.hpp:30-39). */
 typedef struct nb_params {
