@@ -625,6 +625,61 @@ __device__ __forceinline__ void sum_partials(float (&f)[R::ACC], int count,
   }
 }
 
+// A finalize workgroup of 256 threads serves 256 / RUNS particles: with
+// RUNS > 1, wave group r sums run r of each particle's partial sequence (its
+// row partials, then its negated reaction partials: `rows + reacs` partials
+// cut into RUNS contiguous runs), and the first group adds the runs' sums in
+// run order.  The order is fixed either way, so results are deterministic.
+// RUNS = 4 pays where one thread per particle leaves most CUs idle (the
+// tile-pair form: 5.3 vs 10.7 us at n = 12800); with enough particles to
+// fill the chip one run per particle is faster (super-tile finalize at
+// n = 262144: 82 vs 91 us).
+template <class R, int RUNS, class RowAt, class ReacAt>
+__device__ __forceinline__ void run_sum(float (&f)[R::ACC], int rows,
+                                        int reacs, int run, RowAt row_at,
+                                        ReacAt reac_at, int stride) {
+  const int len = rows + reacs;
+  const int k0 = run * len / RUNS, k1 = (run + 1) * len / RUNS;
+#pragma unroll
+  for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
+  if (k0 < rows)
+    sum_partials<R, false>(
+        f, min(k1, rows) - k0, [&](int k) { return row_at(k0 + k); }, stride);
+  const int r0 = max(k0, rows) - rows, r1 = k1 - rows;
+  if (r1 > r0)
+    sum_partials<R, true>(
+        f, r1 - r0, [&](int k) { return reac_at(r0 + k); }, stride);
+}
+
+// First group: f (run 0's sum) plus runs 1 .. RUNS-1 in order.  A run's
+// running sum has the layout and value of a row partial (Kahan: value = sum
+// - compensation), so it is added as one.  Called by every thread (barrier);
+// returns true in the threads that hold the total.
+template <class R, int RUNS>
+__device__ __forceinline__ bool combine_runs(float (&f)[R::ACC],
+                                             float (*red)[256]) {
+  if constexpr (RUNS == 1) {
+    return true;
+  } else {
+    constexpr int PW = 256 / RUNS;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < R::ACC; ++e) red[e][t] = f[e];
+    __syncthreads();
+    if (t >= PW) return false;
+#pragma unroll
+    for (int q = 1; q < RUNS; ++q) {
+      float v[R::ACC];
+#pragma unroll
+      for (int e = 0; e < R::ACC; ++e) v[e] = red[e][t + q * PW];
+      R::add_v(f, v);
+    }
+    return true;
+  }
+}
+
+constexpr int kTileRuns = 4; // runs per particle, tile-pair finalize
+
 template <class R, int U>
 __global__ __launch_bounds__(256) void sym_kernel(SymArgs a) {
   constexpr int P = 64 * U;
@@ -661,7 +716,7 @@ __global__ __launch_bounds__(256) void sym_kernel(SymArgs a) {
 }
 
 // Per particle: row partials (chunk order), minus the negated reaction
-// partials of its column (ga order), then the rule's epilogue.
+// partials of its column (ga order); then the rule's epilogue.
 template <class R, int U>
 __global__ __launch_bounds__(256) void sym_finalize(SymArgs a) {
   constexpr int ST = kWaves * 64 * U;
@@ -670,16 +725,10 @@ __global__ __launch_bounds__(256) void sym_finalize(SymArgs a) {
   const int g = i / ST, o = i % ST;
   const int ts = a.ts, L = a.chunk;
   const long long first = row_first(g, ts, L);
-  const int nc = (ts - g + L - 1) / L;
   float f[R::ACC];
-#pragma unroll
-  for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
-  sum_partials<R, false>(
-      f, nc,
+  run_sum<R, 1>(
+      f, (ts - g + L - 1) / L, g, 0,
       [&](int c) { return a.fpart + (size_t)(first + c) * (R::ACC * ST) + o; },
-      ST);
-  sum_partials<R, true>( // negated reaction sums
-      f, g,
       [&](int ga) {
         return a.spart + (size_t)pair_slot(ga, g, ts) * (R::ACC * ST) + o;
       },
@@ -775,30 +824,34 @@ __global__ __launch_bounds__(256) void sym_tile_kernel(SymArgs a) {
 }
 
 // Per particle: row partials of its tile (B order) minus the negated
-// reaction partials of its column (A order), then the rule's epilogue.
+// reaction partials of its column (A order), in kTileRuns runs; then the
+// rule's epilogue.
 template <class R, int U>
 __global__ __launch_bounds__(256) void sym_tile_finalize(SymArgs a) {
   constexpr int P = 64 * U;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= a.n) return;
-  const int g = i / P, o = i % P, T = a.ts;
+  constexpr int PW = 256 / kTileRuns; // particles per workgroup
+  __shared__ float red[R::ACC][256];
+  const int i = blockIdx.x * PW + threadIdx.x % PW;
+  const int run = threadIdx.x / PW;
+  const bool live = i < a.n;
   float f[R::ACC];
+  if (live) {
+    const int g = i / P, o = i % P, T = a.ts;
+    run_sum<R, kTileRuns>(
+        f, T - g, g, run,
+        [&](int k) {
+          return a.fpart + (size_t)tri_slot(g, g + k, T) * (2 * R::ACC * P) + o;
+        },
+        [&](int c) {
+          return a.fpart + (size_t)tri_slot(c, g, T) * (2 * R::ACC * P) +
+                 R::ACC * P + o;
+        },
+        P);
+  } else {
 #pragma unroll
-  for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
-  sum_partials<R, false>(
-      f, T - g,
-      [&](int k) {
-        return a.fpart + (size_t)tri_slot(g, g + k, T) * (2 * R::ACC * P) + o;
-      },
-      P);
-  sum_partials<R, true>(
-      f, g,
-      [&](int c) {
-        return a.fpart + (size_t)tri_slot(c, g, T) * (2 * R::ACC * P) +
-               R::ACC * P + o;
-      },
-      P);
-  R::finish(a, i, f);
+    for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
+  }
+  if (combine_runs<R, kTileRuns>(f, red) && live) R::finish(a, i, f);
 }
 
 // ---- many independent systems (batched persistent launch) ----------------
@@ -906,20 +959,14 @@ __global__ __launch_bounds__(256) void sym_batch_finalize(SymBatchArgs b) {
   }
   const SymArgs a = system_view(b, lo, ST);
   const int li = i - b.sys_off[lo];
-  const int g = li / ST, o = li % ST;
+  const int g = li / ST, o = li % ST, ts = a.ts;
   const float *base = b.part + (size_t)b.sys_slot[lo] * (2 * R::ACC * ST) + o;
   float f[R::ACC];
-#pragma unroll
-  for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
-  const int ts = a.ts;
-  sum_partials<R, false>(
-      f, ts - g,
+  run_sum<R, 1>(
+      f, ts - g, g, 0,
       [&](int k) {
         return base + (size_t)tri_slot(g, g + k, ts) * (2 * R::ACC * ST);
       },
-      ST);
-  sum_partials<R, true>( // negated reaction sums
-      f, g,
       [&](int ga) {
         return base + (size_t)tri_slot(ga, g, ts) * (2 * R::ACC * ST) +
                R::ACC * ST;
@@ -1011,9 +1058,11 @@ int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
                                     stream),
                     "symmetric pair launch");
   if (r) return r;
+  const int per_block = tiles ? 256 / kTileRuns : 256;
   return hip_check(
       hipLaunchKernel(reinterpret_cast<const void *>(sh->finalize),
-                      dim3((n + 255) / 256), dim3(256), args, 0, stream),
+                      dim3((n + per_block - 1) / per_block), dim3(256), args,
+                      0, stream),
       "symmetric finalize launch");
 }
 
