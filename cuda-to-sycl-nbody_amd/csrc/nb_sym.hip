@@ -768,6 +768,17 @@ __global__ __launch_bounds__(256) void sym_tile_kernel(SymArgs a) {
     A = lo;
     B = A + (int)(item - tri_slot(A, A, T));
   }
+  // targets first: their loads are in flight while the source tile is staged
+  Lane<R, U> L;
+  bool vt[U];
+  const int t0 = A * P + lane * U;
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    R::load(a, t0 + k, L.t[k]);
+    vt[k] = t0 + k < n;
+#pragma unroll
+    for (int e = 0; e < R::ACC; ++e) L.a[k][e] = 0.0f;
+  }
 #pragma unroll
   for (int r = 0; r < U; ++r) {
     const int o = r * 64 + lane;
@@ -779,17 +790,6 @@ __global__ __launch_bounds__(256) void sym_tile_kernel(SymArgs a) {
   }
   __syncthreads(); // the only barrier: every wave has staged its tile
   if (!live) return;
-
-  Lane<R, U> L;
-  bool vt[U];
-  const int t0 = A * P + lane * U;
-#pragma unroll
-  for (int k = 0; k < U; ++k) {
-    R::load(a, t0 + k, L.t[k]);
-    vt[k] = t0 + k < n;
-#pragma unroll
-    for (int e = 0; e < R::ACC; ++e) L.a[k][e] = 0.0f;
-  }
   float *slot = a.fpart + (size_t)item * (2 * R::ACC * P);
   bool masked = false;
   if constexpr (R::MASKED_TAIL) masked = (n % P) != 0 && B == T - 1;
