@@ -322,7 +322,7 @@ int adjoint_default_config(size_t n, bool skip) {
   // 4780 / 6511 vs 4779 / 6478, 262144 19280 / 25892 vs 19168 / 25601
   // (4 per lane 23660 / 31135, 8 per lane 24283 / 34040).  Best
   // target-sharded: 8192 65.8 / 74.2, 262144 38.7 / 43.5 ms.
-  if (!skip && n <= kSymMaxParticles) {
+  if (n <= kSymMaxParticles) { // skip or not: self pairs masked in-kernel
     if (n > kSymSmallMaxParticles) return 14; // super-tile pairs, 4 per lane
     if (n >= 98304) return 20; // super-tile pairs, 2 per lane, 2 per item
     if (n >= 20480) return 19; // super-tile pairs, 2 per lane, 1 per item
@@ -351,11 +351,7 @@ int launch_adjoint(const StepConsts &c, bool skip, bool fused, int cfg,
     return set_error(NB_ERR_INVALID_ARG, "adjoint config out of range");
   if (n <= 0) return NB_OK;
   const FwdConfig &fc = kAdjConfigs[cfg];
-  if (fc.scheme != SCHEME_TARGETS) {
-    if (skip)
-      return set_error(NB_ERR_UNSUPPORTED,
-                       "symmetric launch shapes need dist_eps^3 a normal "
-                       "float");
+  if (fc.scheme != SCHEME_TARGETS) { // self pairs masked in-kernel if skip
     return launch_adjoint_sym(c, fused, fc.unroll, fc.split,
                               fc.scheme == SCHEME_SYMMETRIC_TILES, pos, lxn, lvn,
                               lx, lv, vel, nxt, n, scratch, stream);

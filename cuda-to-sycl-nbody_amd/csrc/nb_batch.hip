@@ -208,10 +208,9 @@ int upload(T **dst, const std::vector<T> &host, const char *what) {
 // table, partial buffer and the persistent grid.
 int configure_sym(nb_batch *b, int cfg) {
   const BatchShape &sh = kBatchShapes[cfg];
-  if (b->mode != MODE_PLAIN)
+  if (b->mode != MODE_PLAIN && b->mode != MODE_PLAIN_SKIP)
     return set_error(NB_ERR_UNSUPPORTED,
-                     "symmetric batch shapes need every system's dist_eps^3 "
-                     "a normal float");
+                     "symmetric batch shapes take BRANCH with plain sums");
   std::vector<int4> items;
   std::vector<int> slots;
   size_t bytes = 0;
@@ -591,8 +590,9 @@ int nb_batch_autotune(nb_batch *b, int trial_steps, int *best_cfg) {
   int best = -1;
   float best_ms = 0.0f;
   for (int cfg = 0; cfg < kNumBatchShapes; ++cfg) {
-    if (kBatchShapes[cfg].scheme == SCHEME_SYMMETRIC && b->mode != MODE_PLAIN)
-      continue; // needs a vanishing self term in every system
+    if (kBatchShapes[cfg].scheme == SCHEME_SYMMETRIC && b->mode != MODE_PLAIN &&
+        b->mode != MODE_PLAIN_SKIP)
+      continue; // BRANCH with plain sums only
     if ((st = configure(b, cfg)) || (st = timed(b, 1, false, false)))
       return st; // warm-up step
     if ((st = copy_state(b, false))) return st;

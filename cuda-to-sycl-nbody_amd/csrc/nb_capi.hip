@@ -37,13 +37,6 @@ int hip_check(hipError_t e, const char *what) {
   return set_error(code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-bool self_term_vanishes(float eps) {
-  // The self term r * d^-3/2 (r = 0, d = eps) is exactly zero when eps^3 is a
-  // positive normal (or infinite) float: then the j == i test can be dropped.
-  const float e3 = (eps * eps) * eps;
-  return e3 >= FLT_MIN; // false for NaN, 0, < 0, subnormal
-}
-
 int pair_mode(int calc_method, int acc, float eps) {
   if (calc_method == NB_CALC_PREDICATED) return MODE_PREDICATED;
   const bool self_zero = self_term_vanishes(eps);

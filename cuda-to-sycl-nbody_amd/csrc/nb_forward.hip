@@ -168,6 +168,10 @@ int forward_config_count() { return kNumConfigs; }
 const FwdConfig &forward_config(int cfg) { return kConfigs[cfg]; }
 
 int forward_default_config(size_t n, int mode) {
+  // the symmetric shapes mask the self pairs where eps^3 is not a normal
+  // float at the cost of the diagonal passes only: same choice
+  if (mode == MODE_PLAIN_SKIP) mode = MODE_PLAIN;
+  if (mode == MODE_KAHAN_SKIP) mode = MODE_KAHAN;
   // Measured on MI355X (disk ICs), us per step.  Tile pairs, 2 per lane:
   // n = 4096 22.6 (target-sharded 128 x 1 split 8: 29), 8192 38.1 (38.7),
   // 12800 57.5 (82.5), 32768 238.4 (323.6; 4 per lane 266.3).  Super-tile
@@ -213,8 +217,10 @@ int forward_default_config(size_t n, int mode) {
 size_t forward_scratch_bytes(int cfg, size_t n, int mode) {
   const FwdConfig &c = kConfigs[cfg];
   if (c.scheme != SCHEME_TARGETS) // (sum, compensation) pairs for Kahan
-    return sym_scratch_bytes(c.unroll, c.split, n, mode == MODE_KAHAN ? 6 : 3,
-                             c.scheme == SCHEME_SYMMETRIC_TILES);
+    return sym_scratch_bytes(
+        c.unroll, c.split, n,
+        mode == MODE_KAHAN || mode == MODE_KAHAN_SKIP ? 6 : 3,
+        c.scheme == SCHEME_SYMMETRIC_TILES);
   return c.split > 1 ? sizeof(float) * 3 * (size_t)c.split * n : 0;
 }
 
@@ -227,17 +233,19 @@ int launch_forward(const StepConsts &c, int mode, int cfg, const float *px,
   if (n <= 0) return NB_OK;
   const FwdConfig &fc = kConfigs[cfg];
   if (fc.scheme != SCHEME_TARGETS) {
-    if (mode != MODE_PLAIN && mode != MODE_KAHAN)
+    // the kernels mask the self pairs themselves when eps^3 is not a normal
+    // float (the SKIP modes)
+    if (mode == MODE_PREDICATED)
       return set_error(NB_ERR_UNSUPPORTED,
-                       "symmetric launch shapes take BRANCH with dist_eps^3 "
-                       "a normal float");
-    if (mode == MODE_KAHAN && fc.unroll > 4)
+                       "symmetric launch shapes take BRANCH only");
+    const bool kahan = mode == MODE_KAHAN || mode == MODE_KAHAN_SKIP;
+    if (kahan && fc.unroll > 4)
       return set_error(NB_ERR_UNSUPPORTED,
                        "symmetric Kahan shapes have 2 or 4 particles per lane");
     return launch_forward_sym(c, fc.unroll, fc.split,
-                              fc.scheme == SCHEME_SYMMETRIC_TILES,
-                              mode == MODE_KAHAN, px, py, pz, vx, vy, vz, nx,
-                              ny, nz, n, scratch, stream);
+                              fc.scheme == SCHEME_SYMMETRIC_TILES, kahan, px,
+                              py, pz, vx, vy, vz, nx, ny, nz, n, scratch,
+                              stream);
   }
   KernelFn fn = find_kernel(fc, mode);
   if (!fn) return set_error(NB_ERR_UNSUPPORTED, "no kernel for this mode");

@@ -7,6 +7,7 @@
 #include "nbody_amd.h"
 
 #include <algorithm>
+#include <cfloat>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -81,7 +82,7 @@ constexpr size_t sym_max_particles(int unroll, bool tiles) {
                : unroll <= 2 ? kSymSmallMaxParticles : kSymMaxParticles;
 }
 
-// Symmetric scheme (nb_sym.hip; vanishing self term only): scratch bytes
+// Symmetric scheme (nb_sym.hip; BRANCH, any eps): scratch bytes
 // for `acc` sums per particle (3 forward and adjoint, 6 fused adjoint) and
 // stream-ordered launches of one forward step / one adjoint step.
 size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc,
@@ -142,12 +143,15 @@ int launch_pack_vec4(const float *x, const float *y, const float *z, float w,
 int pair_mode(int calc_method, int acc, float eps);
 
 // True when the self term of the pair sum is exactly zero (no j == i test
-// needed): dist_eps > 0 with a normal or infinite eps^3.
-bool self_term_vanishes(float eps);
+// needed): the self term r * d^-3/2 (r = 0, d = eps) vanishes when eps^3 is
+// a positive normal (or infinite) float; false for NaN, 0, < 0, subnormal.
+__host__ __device__ inline bool self_term_vanishes(float eps) {
+  const float e3 = (eps * eps) * eps;
+  return e3 >= FLT_MIN;
+}
 
 // Adjoint launch shapes (same fields as the forward ones).  skip: the self
-// term needs masking (dist_eps^3 not a normal float), which rules out the
-// symmetric shapes.
+// term needs masking (dist_eps^3 not a normal float).
 int adjoint_config_count();
 const FwdConfig &adjoint_config(int cfg);
 int adjoint_default_config(size_t n, bool skip);

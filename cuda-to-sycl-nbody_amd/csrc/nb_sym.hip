@@ -3,9 +3,8 @@
 //
 // Two pair rules share one decomposition:
 //   ForceRule    the forward step, particle_interaction<BRANCH>
-//                (src/simulator.cu:186-229), in the parameter range where the
-//                self term vanishes (dist_eps^3 a normal float):
-//                  F_i = sum_j r_ij K_ij,  r_ij = x_j - x_i,
+//                (src/simulator.cu:186-229):
+//                  F_i = sum_{j != i} r_ij K_ij,  r_ij = x_j - x_i,
 //                  K = rsqrt(d^3),  d = |r|^2 + eps  (nb_pairs.hpp arithmetic);
 //   AdjointRule  the reverse-mode step of the same integrator (nb_adjoint.hip
 //                derives it): A_k = sum_j K (h + t3 r),  h = g_j - g_k,
@@ -47,10 +46,14 @@
 // one persistent queue (sym_batch_kernel).
 // Padding (index >= n): ForceRule puts padding particles at 1e30 in every
 // coordinate, so any pair with a real particle has |r|^2 = +inf,
-// rsqrt(inf) = 0 and r * 0 = 0 (exact zeros, no mask).  AdjointRule masks
-// K and t3 of pairs with a padding particle in the items that contain the
-// last, partial (super-)tile (a product r . h of a far padding particle
-// could overflow).
+// rsqrt(inf) = 0 and r * 0 = 0 (exact zeros, no mask; a pair of two padding
+// particles only touches padding sums).  AdjointRule masks K and t3 of pairs
+// with a padding particle in the items that contain the last, partial
+// (super-)tile (a product r . h of a far padding particle could overflow).
+// Self pairs: with dist_eps^3 a normal float the self term is exactly +0
+// (r = 0, finite K); otherwise (eps = 0 or tiny) the pass of each wave over
+// its own tile, the only one holding self pairs, masks them as the
+// reference's `continue` skips them.
 //
 // Per pair term (two interactions): forward 14 VALU + v_rsq_f32 + 3/U DPP
 // moves; adjoint 25 VALU + v_rsq_f32 (K and 1/d from rsqrt(d)) + 3/U DPP;
@@ -160,20 +163,23 @@ struct ForceRule : PlainPartials<3> {
 
   // Source s on target t: a += r K.  With SYM, rs accumulates +r K too: rs
   // holds the negated reaction sum of the source (the finalize subtracts it),
-  // which keeps every accumulation a two-operand v_fmac_f32.
+  // which keeps every accumulation a two-operand v_fmac_f32.  With MASK the
+  // term of a pair that is not `valid` is zero (the self pair when eps^3 is
+  // not a normal float, as the reference's `continue` skips it).
   template <bool SYM, bool MASK>
   static __device__ __forceinline__ void pair(const float (&s)[SRC],
                                               const float (&t)[SRC],
                                               float (&a)[ACC],
                                               float (&rs)[ACC], float eps,
-                                              bool) {
+                                              bool valid) {
     const float rx = s[0] - t[0];
     const float ry = s[1] - t[1];
     const float rz = s[2] - t[2];
     float d = __builtin_fmaf(rx, rx, eps);
     d = __builtin_fmaf(ry, ry, d);
     d = __builtin_fmaf(rz, rz, d);
-    const float K = __builtin_amdgcn_rsqf((d * d) * d);
+    float K = __builtin_amdgcn_rsqf((d * d) * d);
+    if constexpr (MASK) K = valid ? K : 0.0f;
     a[0] = __builtin_fmaf(rx, K, a[0]);
     a[1] = __builtin_fmaf(ry, K, a[1]);
     a[2] = __builtin_fmaf(rz, K, a[2]);
@@ -211,14 +217,15 @@ struct KahanForceRule {
                                               const float (&t)[SRC],
                                               float (&a)[ACC],
                                               float (&rs)[ACC], float eps,
-                                              bool) {
+                                              bool valid) {
     const float rx = s[0] - t[0];
     const float ry = s[1] - t[1];
     const float rz = s[2] - t[2];
     float d = __builtin_fmaf(rx, rx, eps);
     d = __builtin_fmaf(ry, ry, d);
     d = __builtin_fmaf(rz, rz, d);
-    const float K = __builtin_amdgcn_rsqf((d * d) * d);
+    float K = __builtin_amdgcn_rsqf((d * d) * d);
+    if constexpr (MASK) K = valid ? K : 0.0f; // as ForceRule
     kahan_add(rx, K, a[0], a[3]);
     kahan_add(ry, K, a[1], a[4]);
     kahan_add(rz, K, a[2], a[5]);
@@ -389,12 +396,15 @@ struct Lane {
 // R::SRC arrays of ST floats in lds_slot order; src0 = global index of the
 // tile's first particle).  With SYM the reaction sums rotate through the
 // wave.  With MASK, pairs with a padding particle contribute nothing (vt:
-// target validity; sources are valid below index n).
-template <class R, int U, bool SYM, bool MASK>
+// target validity; sources are valid below index n).  With SELF (a wave's
+// pass over its own tile, with MASK) the self pairs contribute nothing
+// either: needed when eps^3 is not a normal float.
+template <class R, int U, bool SYM, bool MASK, bool SELF = false>
 __device__ __forceinline__ void tile_pass(Lane<R, U> &L, const float *src,
                                           int j, int lane, float eps,
                                           const bool (&vt)[U], int src0,
                                           int n) {
+  static_assert(!SELF || (MASK && !SYM), "self pairs: masked own-tile pass");
   constexpr int P = 64 * U;
   constexpr int ST = kWaves * P;
   constexpr int Q = U / 4;
@@ -425,9 +435,13 @@ __device__ __forceinline__ void tile_pass(Lane<R, U> &L, const float *src,
     for (int q = 0; q < U; ++q) {
       const bool vs = !MASK || (src0 + m * U + q < n);
 #pragma unroll
-      for (int k = 0; k < U; ++k)
+      for (int k = 0; k < U; ++k) {
+        // SELF (own tile): at step 0 member q of the lane's own set is its
+        // target q, the pair the reference skips
+        const bool self = SELF && s == 0 && q == k;
         R::template pair<SYM, MASK>(sv[q], L.t[k], L.a[k], L.rs[q], eps,
-                                    vs && vt[k]);
+                                    vs && vt[k] && !self);
+      }
     }
     if constexpr (SYM) {
 #pragma unroll
@@ -452,9 +466,18 @@ __device__ __forceinline__ void super_pair(Lane<R, U> &L, const float *src,
   // 4 per lane the six tile pairs with reactions (4 per lane: 13.13 -> 12.30
   // ms).
   constexpr bool kDiagReactions = U <= 4;
+  // the pass of a wave over its own tile holds the self pairs: +0 when eps^3
+  // is a normal float, masked otherwise
+  const bool self_masked = diag && !self_term_vanishes(eps);
   if (diag && !kDiagReactions) {
+    if (self_masked)
+      tile_pass<R, U, false, true, true>(L, src, w, lane, eps, vt,
+                                         gb0src + w * P, n);
+    else
+      tile_pass<R, U, false, MASK>(L, src, w, lane, eps, vt, gb0src + w * P,
+                                   n);
 #pragma unroll 1
-    for (int b = 0; b < kWaves; ++b) {
+    for (int b = 1; b < kWaves; ++b) {
       const int j = (w + b) & (kWaves - 1);
       tile_pass<R, U, false, MASK>(L, src, j, lane, eps, vt, gb0src + j * P,
                                    n);
@@ -463,7 +486,7 @@ __device__ __forceinline__ void super_pair(Lane<R, U> &L, const float *src,
   }
   if (diag) {
     // Pass 0: each wave against its own tile, without reactions (self terms
-    // are +0), and zeroes that tile's reaction sums.  Pass b > 0: wave w
+    // +0 or masked), and zeroes that tile's reaction sums.  Pass b > 0: wave w
     // meets tile j = w + b only if j < 4, with reactions, so each of the six
     // tile pairs is evaluated once; tile j is visited by one wave per pass.
     // Last, each wave subtracts the negated reactions its own tile received.
@@ -474,7 +497,12 @@ __device__ __forceinline__ void super_pair(Lane<R, U> &L, const float *src,
 #pragma unroll
         for (int c = 0; c < R::ACC; ++c) sacc[c * ST + o + q] = 0.0f;
     }
-    tile_pass<R, U, false, MASK>(L, src, w, lane, eps, vt, gb0src + w * P, n);
+    if (self_masked)
+      tile_pass<R, U, false, true, true>(L, src, w, lane, eps, vt,
+                                         gb0src + w * P, n);
+    else
+      tile_pass<R, U, false, MASK>(L, src, w, lane, eps, vt, gb0src + w * P,
+                                   n);
     __syncthreads();
 #pragma unroll 1
     for (int b = 1; b < kWaves; ++b) {
@@ -794,7 +822,10 @@ __global__ __launch_bounds__(256) void sym_tile_kernel(SymArgs a) {
   bool masked = false;
   if constexpr (R::MASKED_TAIL) masked = (n % P) != 0 && B == T - 1;
   if (A == B) {
-    if (masked)
+    if (!self_term_vanishes(a.c.eps)) // self pairs masked, as the tail
+      tile_pass<R, U, false, true, true>(L, src, w, lane, a.c.eps, vt, B * P,
+                                         n);
+    else if (masked)
       tile_pass<R, U, false, R::MASKED_TAIL>(L, src, w, lane, a.c.eps, vt,
                                              B * P, n);
     else

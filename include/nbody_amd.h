@@ -200,8 +200,8 @@ int nb_forward_config_info(int cfg, int *block, int *unroll, int *tile,
  * target-sharded one):
  *   NB_SCHEME_TARGETS    each thread sums all sources for its targets;
  *   NB_SCHEME_SYMMETRIC  each pair term is evaluated once and applied to both
- *                        particles (Newton's third law); plain accumulation
- *                        with dist_eps^3 a normal float, n <= 2^20 (2^18
+ *                        particles (Newton's third law); BRANCH (plain or
+ *                        Kahan sums), n <= 2^20 (2^18
  *                        with 2 particles per lane), scratch growing as n^2
  *                        (0.4 GB at n = 2^18 with 8 per lane, 4.8 GB at
  *                        2^20); work item = a pair of 4-tile super-tiles;
@@ -248,8 +248,9 @@ int nb_fwd_adjoint_step_dev(const nb_params *p, const nb_vec3 *pos,
 int nb_adjoint_config_count(void);
 int nb_adjoint_config_info(int cfg, int *block, int *unroll, int *tile,
                            int *split);
-/* Scheme of an adjoint launch shape (NB_SCHEME_*; the symmetric shapes need
- * dist_eps^3 a normal float and n <= 2^20), or -1 for a bad index. */
+/* Scheme of an adjoint launch shape (NB_SCHEME_*; the symmetric shapes take
+ * n <= 2^20, 2^18 with 2 particles per lane, 2^17 for tile pairs), or -1 for
+ * a bad index. */
 int nb_adjoint_config_scheme(int cfg);
 
 /* ---- differentiable trajectory with checkpoints (extension) ------------- */
@@ -298,8 +299,7 @@ int nb_batch_config_count(void);
 int nb_batch_config_info(int cfg, int *block, int *unroll, int *tile);
 /* Scheme of a batch shape (NB_SCHEME_TARGETS: work item = a block of one
  * system's targets; NB_SCHEME_SYMMETRIC: work item = a super-tile pair of one
- * system, each pair term evaluated once, for batches whose systems all have
- * dist_eps^3 a normal float), or -1 for a bad index. */
+ * system, each pair term evaluated once), or -1 for a bad index. */
 int nb_batch_config_scheme(int cfg);
 float nb_batch_last_ms(const nb_batch *b);
 size_t nb_batch_num_particles(const nb_batch *b);

@@ -77,6 +77,40 @@ def test_adjoint_pair_sum_vs_fp64(nb, orc, gpu, which):
         assert np.array_equal(lv.cpu().numpy(), np.float32(0.5) * g), info
 
 
+def test_adjoint_vanishing_eps(nb, orc, gpu):
+    """dist_eps = 0: every symmetric adjoint shape (and the fused step)
+    masks the self pairs; the pair sum matches the fp64 adjoint, which skips
+    j == k."""
+    import torch
+
+    n = 3000
+    pos, vel = nb.plummer_ic(n, seed=23, g_mass=1.0)
+    rng = np.random.default_rng(5)
+    g = rng.normal(size=(3, n)).astype(np.float32)
+    params = nb.default_params(num_particles=n, dt=1.0, G=1.0, damping=0.5, dist_eps=0.0)
+    p = orc.params(dt=1.0, G=1.0, damping=0.5, dist_eps=0.0)
+    ref_lx, _ = orc.adjoint_step_f64(p, pos, np.zeros((3, n)), g.astype(np.float64))
+    ref = np.stack(ref_lx)
+    P = to_dev(torch, pos, gpu)
+    V = to_dev(torch, vel, gpu)
+    lxn = torch.zeros(3, n, device=gpu)
+    lvn = torch.from_numpy(g).to(gpu)
+    for cfg in [-1] + sym_adjoint_configs(nb):
+        for fused in (False, True):
+            lx = torch.empty(3, n, device=gpu)
+            lv = torch.empty(3, n, device=gpu)
+            if fused:
+                nb.fwd_adjoint_step(params, P, V.clone(), torch.empty_like(P), lxn, lvn,
+                                    lx, lv, cfg=cfg)
+            else:
+                nb.adjoint_step(params, P, lxn, lvn, lx, lv, cfg=cfg)
+            torch.cuda.synchronize()
+            got = lx.cpu().numpy().astype(np.float64)
+            what = (cfg, fused)
+            assert np.isfinite(got).all(), what
+            assert rel(got, ref) < 1e-5, (what, rel(got, ref))
+
+
 @pytest.mark.parametrize("n", [4096, 33000])
 def test_adjoint_step_default_params(nb, orc, gpu, n):
     """SimParam defaults, the library's shape (tile pairs at 4096, super-tile

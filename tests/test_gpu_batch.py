@@ -218,12 +218,30 @@ def test_symmetric_batch_vs_oracle(nb, orc, gpu):
     assert np.percentile(e, 99) <= 1e-5 and e.max() <= 1e-4
 
 
-def test_symmetric_batch_needs_vanishing_self_term(nb, gpu):
-    params, pos, vel = make_batch(nb, [300, 400])
+def test_symmetric_batch_vanishing_eps(nb, gpu):
+    """A system with dist_eps = 0 in a symmetric batch: self pairs masked,
+    results equal to the single-system symmetric step bit for bit."""
+    import torch
+
+    sizes = [300, 1500]
+    params, pos, vel = make_batch(nb, sizes)
     params[1].dist_eps = 0.0
-    b = nb.Batch(params, pos, vel)
-    with pytest.raises(nb.NbError, match="symmetric"):
-        b.set_config(sym_batch_configs(nb)[0])
-    best = b.autotune(1)  # skips the symmetric shapes
-    assert nb.batch_config_info(best)["scheme"] == "targets"
-    b.close()
+    for bcfg in sym_batch_configs(nb):
+        fcfg = forward_twin(nb, bcfg)
+        b = nb.Batch(params, pos, vel)
+        b.set_config(bcfg)
+        b.step(1)
+        bp, bv = b.positions(), b.velocities()
+        b.close()
+        off = 0
+        for p, n in zip(params, sizes):
+            dp = [torch.from_numpy(a[off:off + n].copy()).to(gpu) for a in pos]
+            dv = [torch.from_numpy(a[off:off + n].copy()).to(gpu) for a in vel]
+            dn = [torch.empty_like(t) for t in dp]
+            nb.forward_step(p, dp, dv, dn, cfg=fcfg)
+            torch.cuda.synchronize()
+            for c in range(3):
+                assert np.isfinite(bp[c][off:off + n]).all(), (bcfg, n)
+                assert np.array_equal(bp[c][off:off + n], dn[c].cpu().numpy()), (bcfg, n)
+                assert np.array_equal(bv[c][off:off + n], dv[c].cpu().numpy()), (bcfg, n)
+            off += n
