@@ -40,7 +40,8 @@ __all__ = [
 
 
 def _ptr(a: np.ndarray) -> ctypes.c_void_p:
-    assert a.dtype == np.float32 and a.flags.c_contiguous
+    if a.dtype != np.float32 or not a.flags.c_contiguous:
+        raise ValueError("expected a contiguous float32 array")
     return ctypes.c_void_p(a.ctypes.data)
 
 
@@ -115,8 +116,9 @@ class Simulator:
             raise ValueError("each array must hold params.num_particles values")
         self._lib = load()
         self._h = ctypes.c_void_p()
-        self.params = params
-        check(self._lib.nb_sim_create(ctypes.byref(params), *[_ptr(a) for a in arrs],
+        # a copy: set_frame_iterations updates it, not the caller's struct
+        self.params = NbParams.from_buffer_copy(params)
+        check(self._lib.nb_sim_create(ctypes.byref(self.params), *[_ptr(a) for a in arrs],
                                       ctypes.byref(self._h)))
 
     def close(self) -> None:

@@ -16,7 +16,8 @@ from ._lib import NB_ERR_INVALID_ARG, NbParams, check, load
 
 
 def _p(a: np.ndarray) -> ctypes.c_void_p:
-    assert a.dtype == np.float32 and a.flags.c_contiguous
+    if a.dtype != np.float32 or not a.flags.c_contiguous:
+        raise ValueError("expected a contiguous float32 array")
     return ctypes.c_void_p(a.ctypes.data)
 
 
