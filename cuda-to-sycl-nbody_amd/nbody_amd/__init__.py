@@ -228,7 +228,7 @@ def pack_vec4(soa, w: float = 1.0, stream=None):
 
     n = int(soa[0].numel())
     out = torch.empty(n, 4, dtype=torch.float32, device=soa.device)
-    check(load().nb_pack_vec4_dev(ctypes.byref(_vec3(soa)), w, ctypes.c_void_p(out.data_ptr()),
+    check(load().nb_pack_vec4_dev(ctypes.byref(_vec3(soa, n)), w, ctypes.c_void_p(out.data_ptr()),
                                   n, _stream(stream)))
     return out
 

@@ -16,12 +16,17 @@ import ctypes
 from ._lib import NB_ERR_INVALID_ARG, NbParams, NbVec3, check, load
 
 
-def _vec3(t) -> NbVec3:
-    """A (3, n) contiguous float32 CUDA tensor, or a sequence of 3 tensors."""
+def _vec3(t, n: int) -> NbVec3:
+    """A (3, n) contiguous float32 CUDA tensor, or a sequence of 3 tensors,
+    each row holding at least n values."""
+    if len(t) != 3:
+        raise ValueError("expected 3 rows (x, y, z)")
     rows = [t[0], t[1], t[2]]
     for r in rows:
         if not r.is_cuda or str(r.dtype) != "torch.float32" or not r.is_contiguous():
             raise ValueError("expected contiguous float32 CUDA tensors")
+        if r.numel() < n:
+            raise ValueError(f"a row holds {r.numel()} < {n} values")
     return NbVec3(*[ctypes.c_void_p(r.data_ptr()) for r in rows])
 
 
@@ -36,7 +41,7 @@ def adjoint_step(params: NbParams, pos, lam_x_next, lam_v_next, lam_x, lam_v, cf
                  stream=None) -> None:
     """Cotangents of (x_n, v_n) from those of (x_{n+1}, v_{n+1}) at positions pos."""
     n = int(pos[0].numel())
-    vs = [_vec3(t) for t in (pos, lam_x_next, lam_v_next, lam_x, lam_v)]
+    vs = [_vec3(t, n) for t in (pos, lam_x_next, lam_v_next, lam_x, lam_v)]
     check(load().nb_adjoint_step_dev(ctypes.byref(params), *[ctypes.byref(v) for v in vs], n,
                                      cfg, _stream(stream)))
 
@@ -45,7 +50,7 @@ def fwd_adjoint_step(params: NbParams, pos, vel, pos_next, lam_x_next, lam_v_nex
                      cfg: int = -1, stream=None) -> None:
     """Forward step (vel in place, pos_next written) and adjoint step at pos, one launch."""
     n = int(pos[0].numel())
-    vs = [_vec3(t) for t in (pos, vel, pos_next, lam_x_next, lam_v_next, lam_x, lam_v)]
+    vs = [_vec3(t, n) for t in (pos, vel, pos_next, lam_x_next, lam_v_next, lam_x, lam_v)]
     check(load().nb_fwd_adjoint_step_dev(ctypes.byref(params), *[ctypes.byref(v) for v in vs],
                                          n, cfg, _stream(stream)))
 
@@ -118,6 +123,11 @@ _FN = None
 def trajectory(x0, v0, params: NbParams, steps: int, every: int = 20):
     """Differentiable T-step trajectory: returns (x_T, v_T), each (3, n)."""
     global _FN
+    for t in (x0, v0):
+        if not t.is_cuda or t.dim() != 2 or t.shape[0] != 3:
+            raise ValueError("x0 and v0 must be (3, n) CUDA tensors")
+    if x0.shape != v0.shape or x0.device != v0.device:
+        raise ValueError("x0 and v0 must have the same shape and device")
     if _FN is None:
         _FN = _make_fn()
     return _FN.apply(x0, v0, params, int(steps), int(every))

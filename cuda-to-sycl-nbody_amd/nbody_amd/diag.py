@@ -15,10 +15,12 @@ def forces_f64(params: NbParams, pos, idx, stream=None):
     import torch
 
     n = int(pos[0].numel())
+    if not idx.is_cuda:
+        raise ValueError("idx must be a CUDA tensor")
     idx = idx.to(torch.int32).contiguous()
     m = int(idx.numel())
     out = torch.empty(3, m, dtype=torch.float64, device=pos.device)
-    check(load().nb_forces_f64_dev(ctypes.byref(params), ctypes.byref(_vec3(pos)),
+    check(load().nb_forces_f64_dev(ctypes.byref(params), ctypes.byref(_vec3(pos, n)),
                                    ctypes.c_void_p(idx.data_ptr()), m,
                                    *[ctypes.c_void_p(out[c].data_ptr()) for c in range(3)], n,
                                    _stream(stream)))
@@ -31,8 +33,8 @@ def energy_momentum(params: NbParams, pos, vel, stream=None) -> dict:
 
     n = int(pos[0].numel())
     out = torch.empty(5, dtype=torch.float64, device=pos.device)
-    check(load().nb_energy_momentum_dev(ctypes.byref(params), ctypes.byref(_vec3(pos)),
-                                        ctypes.byref(_vec3(vel)), n,
+    check(load().nb_energy_momentum_dev(ctypes.byref(params), ctypes.byref(_vec3(pos, n)),
+                                        ctypes.byref(_vec3(vel, n)), n,
                                         ctypes.c_void_p(out.data_ptr()), _stream(stream)))
     k, w, px, py, pz = out.cpu().tolist()
     return {"kinetic": k, "potential": w, "total": k + w, "momentum": (px, py, pz)}
