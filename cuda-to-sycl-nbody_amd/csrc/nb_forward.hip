@@ -180,8 +180,10 @@ int forward_default_config(size_t n, int mode) {
   // 1860, 8 per lane 1915), 131072 3118 / 3104 (4 per lane 3176, 8 per lane
   // 3299), 262144 12236 / 12133 (8 per lane 11804).  Past 2^19 two
   // super-tiles of 8 per lane per item run as fast as one and need 25 %
-  // less scratch (6042 Gint/s at 2^20).
-  if (mode == MODE_PLAIN && n <= kSymMaxParticles) {
+  // less scratch (6042 Gint/s at 2^20); past 2^20 four per item keep the
+  // partials at 16 GB for 2^21 particles.
+  if (mode == MODE_PLAIN && n <= kSymLargeMaxParticles) {
+    if (n > kSymMaxParticles) return 24;   // symmetric, 8 per lane, 4 per item
     if (n > ((size_t)1 << 19)) return 23; // symmetric, 8 per lane, 2 per item
     if (n >= ((size_t)1 << 18)) return 22; // symmetric, 8 per lane, 1 per item
     if (n >= 98304) return 35;             // symmetric, 2 per lane, 2 per item

@@ -76,10 +76,18 @@ constexpr size_t kSymTileMaxParticles = (size_t)1 << 17;
 // (1.6 GB of partials at 2^18).
 constexpr size_t kSymSmallMaxParticles = (size_t)1 << 18;
 
-// Largest particle count of a symmetric launch shape.
-constexpr size_t sym_max_particles(int unroll, bool tiles) {
-  return tiles ? kSymTileMaxParticles
-               : unroll <= 2 ? kSymSmallMaxParticles : kSymMaxParticles;
+// Largest particle count of the plain-sum shapes with 8 particles per lane
+// and at least four super-tiles per work item (16 GB of partials at 2^21).
+constexpr size_t kSymLargeMaxParticles = (size_t)1 << 21;
+
+// Largest particle count of a symmetric launch shape (acc: sums per
+// particle, 3 plain, 6 for Kahan and the fused adjoint).
+constexpr size_t sym_max_particles(int unroll, bool tiles, int chunk = 1,
+                                   int acc = 3) {
+  return tiles         ? kSymTileMaxParticles
+         : unroll <= 2 ? kSymSmallMaxParticles
+         : (unroll == 8 && chunk >= 4 && acc == 3) ? kSymLargeMaxParticles
+                                                   : kSymMaxParticles;
 }
 
 // Symmetric scheme (nb_sym.hip; BRANCH, any eps): scratch bytes

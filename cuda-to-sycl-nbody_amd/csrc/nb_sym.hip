@@ -1054,12 +1054,16 @@ const SymShape *find_kahan_shape(int unroll, bool tiles) {
 
 int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
                float *scratch, int acc, bool tiles, hipStream_t stream) {
-  if ((size_t)n > sym_max_particles(unroll, tiles))
+  const size_t cap = sym_max_particles(unroll, tiles, chunk, acc);
+  if ((size_t)n > cap)
     return set_error(
         NB_ERR_UNSUPPORTED,
         tiles         ? "symmetric tile-pair shapes take at most 2^17 particles"
         : unroll <= 2 ? "symmetric shapes with 2 per lane take at most 2^18 particles"
-                      : "symmetric launch shapes take at most 2^20 particles");
+        : cap == kSymLargeMaxParticles
+            ? "symmetric launch shapes take at most 2^21 particles"
+            : "symmetric launch shapes take at most 2^20 particles (2^21 with "
+              "plain sums, 8 per lane and 4 super-tiles per item)");
   if (!sh || chunk < 1)
     return set_error(NB_ERR_UNSUPPORTED, "no symmetric kernel for this shape");
   if (!scratch)
@@ -1102,12 +1106,13 @@ int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
 size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc,
                          bool tiles) {
   if (tiles) {
-    if (n == 0 || n > sym_max_particles(unroll, tiles)) return 0;
+    if (n == 0 || n > sym_max_particles(unroll, tiles, chunk, acc)) return 0;
     const long long P = 64LL * unroll;
     const long long T = ((long long)n + P - 1) / P;
     return sizeof(float) * (size_t)(T * (T + 1) / 2) * 2 * acc * (size_t)P;
   }
-  if (n == 0 || n > sym_max_particles(unroll, tiles) || chunk < 1) return 0;
+  if (n == 0 || n > sym_max_particles(unroll, tiles, chunk, acc) || chunk < 1)
+    return 0;
   const long long st = (long long)kWaves * 64 * unroll;
   const long long ts = ((long long)n + st - 1) / st;
   const long long slots = items_upto(ts, chunk) + ts * (ts - 1) / 2;
