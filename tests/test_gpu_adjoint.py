@@ -263,6 +263,26 @@ def test_trajectory_gradient_vs_fp64_same_states(nb, orc, gpu):
     assert ex <= 1e-3 and ev <= 1e-3
 
 
+def test_trajectory_gradient_zero_eps(nb, orc, gpu):
+    """dist_eps = 0 through the whole differentiable trajectory (symmetric
+    forward and adjoint shapes, self pairs masked): the checkpointed adjoint
+    against the fp64 adjoint (which skips j == k) along the same states."""
+    import torch
+
+    n, T, K = 4096, 10, 5
+    x0, v0, params, target = disk_problem(nb, orc, torch, gpu, n, T, K, eps=0.0)
+    L, gx, gv, xT = gpu_gradient(nb, torch, x0, v0, params, target, T, K)
+    states = gpu_states(nb, torch, params, x0, v0, T)
+    assert np.array_equal(states[-1], xT.detach().cpu().numpy())
+    assert np.isfinite(gx).all() and np.isfinite(gv).all()
+    gx64, gv64 = fp64_adjoint_along(orc, orc.params(dist_eps=0.0), states,
+                                    target.double().cpu().numpy())
+    ex, ev = rel(gx, gx64), rel(gv, gv64)
+    print(f"N={n} T={T} K={K} eps=0: loss {L:.6e}; grad vs fp64 adjoint on the same "
+          f"states: x {ex:.2e} v {ev:.2e}")
+    assert ex <= 1e-3 and ev <= 1e-3
+
+
 def test_trajectory_gradient_vs_fp64_trajectory(nb, orc, gpu):
     """Independent fp64 trajectory + adjoint (every state kept), for a loss
     whose residual is far above fp32 rounding."""
