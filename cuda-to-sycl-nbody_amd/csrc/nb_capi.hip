@@ -99,6 +99,7 @@ struct nb_sim {
   int cfg_req = -1; // shape asked for with nb_sim_set_config (-1: default)
   float last_ms = 0.0f, last_kernel_ms = 0.0f;
   float *staging = nullptr; // pinned host copy of pos and vel (6 arrays)
+  float *staging_dev = nullptr; // the same memory, device address
   hipEvent_t ev_stage = nullptr;
   bool staged = false;
   // Frames of two or more steps replay a captured hipGraph, one per start
@@ -413,26 +414,29 @@ int nb_sim_stage_readback(nb_sim *s) {
   }
   int st = NB_OK;
   if (!s->staging) {
-    void *p = nullptr;
+    // pinned, coherent and mapped into the device's address space: the
+    // staging kernel stores straight into it
+    void *p = nullptr, *d = nullptr;
     if ((st = hip_check(hipHostMalloc(&p, sizeof(float) * 6 * s->stride,
-                                      hipHostMallocDefault),
+                                      hipHostMallocMapped),
                         "pinned hipHostMalloc")))
       return st;
     s->staging = static_cast<float *>(p);
+    if ((st = hip_check(hipHostGetDevicePointer(&d, p, 0),
+                        "hipHostGetDevicePointer")))
+      return st;
+    s->staging_dev = static_cast<float *>(d);
   }
   if (!s->ev_stage &&
       (st = hip_check(hipEventCreateWithFlags(&s->ev_stage,
                                               hipEventDisableTiming),
                       "hipEventCreate")))
     return st;
-  float *const src[6] = {s->pos[0], s->pos[1], s->pos[2],
-                         s->vel[0], s->vel[1], s->vel[2]};
-  for (int c = 0; c < 6; ++c)
-    if ((st = hip_check(hipMemcpyAsync(s->staging + c * s->stride, src[c],
-                                       sizeof(float) * s->n,
-                                       hipMemcpyDeviceToHost, s->stream),
-                        "staged device-to-host copy")))
-      return st;
+  const float *const src[6] = {s->pos[0], s->pos[1], s->pos[2],
+                               s->vel[0], s->vel[1], s->vel[2]};
+  if ((st = launch_stage_six(src, s->staging_dev, (int)s->n, s->stride,
+                             s->stream)))
+    return st;
   if ((st = hip_check(hipEventRecord(s->ev_stage, s->stream), "hipEventRecord")))
     return st;
   s->staged = true;

@@ -144,6 +144,10 @@ size_t forward_scratch_bytes(int cfg, size_t n, int mode);
 float *device_scratch(size_t bytes, int *status);
 
 // SoA -> AoS vec4 pack on the device (out: 4 n floats, 16 B aligned).
+// Copies six device arrays of n floats to dst + c * stride (c = 0..5), a
+// device pointer to host-mapped pinned memory, in one launch on `stream`.
+int launch_stage_six(const float *const src[6], float *dst, int n,
+                     size_t stride, hipStream_t stream);
 int launch_pack_vec4(const float *x, const float *y, const float *z, float w,
                      float *out, int n, hipStream_t stream);
 

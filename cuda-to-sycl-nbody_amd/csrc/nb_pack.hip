@@ -11,6 +11,10 @@
 namespace nb {
 namespace {
 
+struct Six {
+  const float *p[6];
+};
+
 __global__ __launch_bounds__(256) void pack_vec4(const float *__restrict__ x,
                                                  const float *__restrict__ y,
                                                  const float *__restrict__ z,
@@ -22,7 +26,32 @@ __global__ __launch_bounds__(256) void pack_vec4(const float *__restrict__ x,
     out[i] = make_float4(x[i], y[i], z[i], w);
 }
 
+// Six device arrays into one host-mapped pinned buffer (row c at dst + c *
+// stride): the readback of a frame's positions and velocities as one launch
+// whose stores cross the bus, instead of six DMA copies whose set-up time
+// dominates at small n.
+__global__ __launch_bounds__(256) void stage_six(Six src,
+                                                 float *__restrict__ dst,
+                                                 int n, long long stride) {
+  const int c = blockIdx.y;
+  const float *__restrict__ a = src.p[c];
+  float *__restrict__ d = dst + c * stride;
+  const int step = gridDim.x * 256;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += step) d[i] = a[i];
+}
+
 } // namespace
+
+int launch_stage_six(const float *const src[6], float *dst, int n,
+                     size_t stride, hipStream_t stream) {
+  if (n <= 0) return NB_OK;
+  Six s;
+  for (int c = 0; c < 6; ++c) s.p[c] = src[c];
+  const int blocks = std::min((n + 255) / 256, 256 * 2);
+  hipLaunchKernelGGL(stage_six, dim3(blocks, 6), dim3(256), 0, stream, s, dst,
+                     n, (long long)stride);
+  return hip_check(hipGetLastError(), "staging launch");
+}
 
 int launch_pack_vec4(const float *x, const float *y, const float *z, float w,
                      float *out, int n, hipStream_t stream) {

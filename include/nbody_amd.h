@@ -129,12 +129,12 @@ int nb_sim_sync(nb_sim *s);
 int nb_sim_read_pos(nb_sim *s, float *x, float *y, float *z);
 int nb_sim_read_vel(nb_sim *s, float *x, float *y, float *z);
 
-/* Readback through pinned host memory (SURVEY's per-frame readback path):
- * nb_sim_stage_readback enqueues asynchronous copies of the current
- * positions and velocities into a pinned buffer owned by the handle and
- * returns at once; nb_sim_staged waits for them and returns six pointers
- * (x, y, z, vx, vy, vz) valid until the next stage call or nb_sim_destroy.
- * The reference issues six synchronous pageable copies per frame instead
+/* Per-frame readback through pinned host memory: nb_sim_stage_readback
+ * enqueues one kernel that stores the current positions and velocities into
+ * a pinned, device-mapped buffer owned by the handle and returns at once;
+ * nb_sim_staged waits for it and returns six pointers (x, y, z, vx, vy, vz)
+ * valid until the next stage call or nb_sim_destroy.  The reference issues
+ * six synchronous pageable copies per frame instead
  * (src/simulator.cu:106-129). */
 int nb_sim_stage_readback(nb_sim *s);
 int nb_sim_staged(nb_sim *s, const float **arrays);

@@ -32,8 +32,8 @@ def orc():
 
 @pytest.fixture(scope="session")
 def gpu(nb):
-    """Skips only when no GPU is visible; on a GPU box the HIP library must
-    load and run (no fallback)."""
+    """Skips only when no GPU is visible; on a machine with a GPU the HIP
+    library must load and run (no fallback)."""
     import torch
 
     if not torch.cuda.is_available():
