@@ -358,8 +358,10 @@ def test_reference_driver_runs(nb, gpu):
             assert len(lines) == 3 and lines[-1].startswith("At step 5 kernel time is ")
 
 
-def test_staged_readback_matches_plain_reads(nb, gpu):
-    n = 3001
+@pytest.mark.parametrize("n", [1, 3001, 300001])
+def test_staged_readback_matches_plain_reads(nb, gpu, n):
+    """The staging kernel (grid-stride loop over six rows into device-mapped
+    pinned memory) returns what the plain copies return, bit for bit."""
     pos, vel = nb.disk_ic(n)
     sim = nb.Simulator(nb.default_params(num_particles=n, sim_iterations_per_frame=3), pos, vel)
     sim.step_sim()
