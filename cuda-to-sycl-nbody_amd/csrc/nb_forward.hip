@@ -181,9 +181,10 @@ int forward_default_config(size_t n, int mode) {
   // 3299), 262144 12236 / 12133 (8 per lane 11804).  Past 2^19 two
   // super-tiles of 8 per lane per item run as fast as one and need 25 %
   // less scratch (6042 Gint/s at 2^20); past 2^20 four per item keep the
-  // partials at 16 GB for 2^21 particles.
-  if (mode == MODE_PLAIN && n <= kSymLargeMaxParticles) {
-    if (n > kSymMaxParticles) return 24;   // symmetric, 8 per lane, 4 per item
+  // partials at 16 GB for 2^21 particles, and past 2^21 the step runs in
+  // bands of source super-tiles with at most 8 GB of partials each.
+  if (mode == MODE_PLAIN && n <= kSymBandedMaxParticles) {
+    if (n > kSymMaxParticles) return 24; // 8 per lane, 4 per item (banded past 2^21)
     if (n > ((size_t)1 << 19)) return 23; // symmetric, 8 per lane, 2 per item
     if (n >= ((size_t)1 << 18)) return 22; // symmetric, 8 per lane, 1 per item
     if (n >= 98304) return 35;             // symmetric, 2 per lane, 2 per item

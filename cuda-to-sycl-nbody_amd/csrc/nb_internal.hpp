@@ -80,7 +80,13 @@ constexpr size_t kSymSmallMaxParticles = (size_t)1 << 18;
 // and at least four super-tiles per work item (16 GB of partials at 2^21).
 constexpr size_t kSymLargeMaxParticles = (size_t)1 << 21;
 
-// Largest particle count of a symmetric launch shape (acc: sums per
+// Past kSymLargeMaxParticles the plain forward shapes with 8 per lane and at
+// least four super-tiles per item run in bands of source super-tiles whose
+// partials take at most kSymBandBytes, up to kSymBandedMaxParticles.
+constexpr size_t kSymBandedMaxParticles = (size_t)1 << 24;
+constexpr size_t kSymBandBytes = (size_t)8 << 30;
+
+// Largest particle count of a single symmetric launch (acc: sums per
 // particle, 3 plain, 6 for Kahan and the fused adjoint).
 constexpr size_t sym_max_particles(int unroll, bool tiles, int chunk = 1,
                                    int acc = 3) {

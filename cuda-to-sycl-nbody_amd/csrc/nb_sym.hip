@@ -116,6 +116,10 @@ struct SymArgs {
   int ts;    // super-tiles
   int chunk; // super-tiles per work item
   StepConsts c;
+  // banded launches (sym_band_*): source super-tiles [b0, b0 + bw) of this
+  // band, first band flag, running per-particle sums [3][n]
+  int b0, bw, first;
+  float *acc3;
 };
 
 // Finalize arithmetic of the plain rules: row partials are added, the
@@ -764,6 +768,116 @@ __global__ __launch_bounds__(256) void sym_finalize(SymArgs a) {
   R::finish(a, i, f);
 }
 
+// ---- banded launches (n past the single-launch limit) ----------------------
+//
+// The super-tile pair triangle is cut by source super-tile into bands
+// [B0, B0 + W): band rows GA < B0 + W, each with the sources GB in
+// [max(GA, B0), B0 + W) in chunks of L.  Per band: the pair kernel writes the
+// band's row partials (one per item) and reaction partials (one per pair
+// with GB > GA), a fold kernel adds, per particle, its band rows minus its
+// band reactions to a running sum (band order), and after the last band a
+// finish kernel integrates.  The partials of one band stay within a fixed
+// budget whatever n is; the order of every sum is fixed.
+
+// Reaction slot of pair (ga, gb), gb > ga, gb in the band [b0, b0 + w).
+__host__ __device__ inline long long band_pair_slot(int ga, int gb, int b0,
+                                                    int w) {
+  return ga < b0 ? (long long)ga * w + (gb - b0)
+                 : (long long)b0 * w + pair_slot(ga - b0, gb - b0, w);
+}
+
+__host__ __device__ inline long long band_items(int b0, int w, int L) {
+  return (long long)b0 * ((w + L - 1) / L) + items_upto(w, L);
+}
+
+__host__ __device__ inline long long band_reactions(int b0, int w) {
+  return (long long)b0 * w + (long long)w * (w - 1) / 2;
+}
+
+template <class R, int U>
+__global__ __launch_bounds__(256) void sym_band_kernel(SymArgs a) {
+  constexpr int P = 64 * U;
+  constexpr int ST = kWaves * P;
+  __shared__ float4 src4[R::SRC][ST / 4];
+  __shared__ float4 sacc4[R::ACC][ST / 4];
+  float *src = reinterpret_cast<float *>(src4);
+  float *sacc = reinterpret_cast<float *>(sacc4);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int L = a.chunk, b0 = a.b0, W = a.bw, b1 = b0 + W;
+  const int c0 = (W + L - 1) / L; // items of a row above the band
+  const long long head = (long long)b0 * c0;
+  const long long it = blockIdx.x;
+  int ga, gb0;
+  if (it < head) {
+    ga = (int)(it / c0);
+    gb0 = b0 + (int)(it % c0) * L;
+  } else { // the band's own triangle, rows enumerated in order
+    const long long j = it - head;
+    int lo = 0, hi = W - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (row_first(mid, W, L) <= j) lo = mid;
+      else hi = mid - 1;
+    }
+    ga = b0 + lo;
+    gb0 = ga + (int)(j - row_first(lo, W, L)) * L;
+  }
+  const int gb1 = min(gb0 + L, b1);
+  Lane<R, U> T;
+  bool vt[U];
+  load_targets(T, vt, a, ga, w, lane);
+  for (int gb = gb0; gb < gb1; ++gb)
+    pair_step(T, vt, a, src, sacc, ga, gb,
+              gb > ga ? a.spart + (size_t)band_pair_slot(ga, gb, b0, W) *
+                                      (R::ACC * ST)
+                      : nullptr);
+  store_row(T, a.fpart + (size_t)it * (R::ACC * ST), w, lane);
+}
+
+// Per particle: its band rows (chunk order) minus its band reactions (GA
+// order), added to the running sum.
+template <class R, int U>
+__global__ __launch_bounds__(256) void sym_band_fold(SymArgs a) {
+  constexpr int ST = kWaves * 64 * U;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  const int g = i / ST, o = i % ST;
+  const int L = a.chunk, b0 = a.b0, W = a.bw, b1 = b0 + W;
+  float f[R::ACC];
+#pragma unroll
+  for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
+  if (g < b1) {
+    const int c0 = (W + L - 1) / L;
+    const long long first =
+        g < b0 ? (long long)g * c0
+               : (long long)b0 * c0 + row_first(g - b0, W, L);
+    const int rows = g < b0 ? c0 : (b1 - g + L - 1) / L;
+    run_sum<R, 1>(
+        f, rows, g >= b0 ? g : 0, 0,
+        [&](int c) { return a.fpart + (size_t)(first + c) * (R::ACC * ST) + o; },
+        [&](int gs) {
+          return a.spart +
+                 (size_t)band_pair_slot(gs, g, b0, W) * (R::ACC * ST) + o;
+        },
+        ST);
+  }
+#pragma unroll
+  for (int e = 0; e < R::ACC; ++e) {
+    float *acc = a.acc3 + (size_t)e * a.n + i;
+    *acc = a.first ? f[e] : *acc + f[e];
+  }
+}
+
+template <class R, int U>
+__global__ __launch_bounds__(256) void sym_band_finish(SymArgs a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  float f[R::ACC];
+#pragma unroll
+  for (int e = 0; e < R::ACC; ++e) f[e] = a.acc3[(size_t)e * a.n + i];
+  R::finish(a, i, f);
+}
+
 // ---- tile-pair items (small n) ----------------------------------------------
 //
 // Work item = one tile pair (A, B), A <= B, of P = 64 U particles, owned by
@@ -1052,8 +1166,83 @@ const SymShape *find_kahan_shape(int unroll, bool tiles) {
   return nullptr;
 }
 
+// Banded plain forward step: the shape (8 per lane, >= 4 super-tiles per
+// item) and size range it serves.
+bool sym_banded(int unroll, bool tiles, int chunk, int acc, size_t n) {
+  return !tiles && unroll == 8 && chunk >= 4 && acc == 3 &&
+         n > kSymLargeMaxParticles && n <= kSymBandedMaxParticles;
+}
+
+// Band width (the widest whose partials fit kSymBandBytes) and the largest
+// row-item and reaction slot counts over the bands.
+struct BandPlan {
+  int ts, w;
+  long long rows, reacs;
+};
+
+BandPlan band_plan(size_t n, int unroll, int chunk, int acc) {
+  const int st = kWaves * 64 * unroll;
+  const int ts = (int)((n + st - 1) / st);
+  const size_t per_slot = sizeof(float) * acc * st;
+  for (int w = ts; w >= 1; --w) {
+    long long rows = 0, reacs = 0;
+    for (int b0 = 0; b0 < ts; b0 += w) {
+      const int ww = std::min(w, ts - b0);
+      rows = std::max(rows, band_items(b0, ww, chunk));
+      reacs = std::max(reacs, band_reactions(b0, ww));
+    }
+    if ((size_t)(rows + reacs) * per_slot <= kSymBandBytes)
+      return {ts, w, rows, reacs};
+  }
+  return {ts, 0, 0, 0};
+}
+
+int launch_sym_banded(SymArgs &a, int chunk, int n, float *scratch,
+                      hipStream_t stream) {
+  using R = ForceRule;
+  constexpr int U = 8;
+  constexpr int ST = kWaves * 64 * U;
+  const BandPlan bp = band_plan((size_t)n, U, chunk, R::ACC);
+  if (bp.w < 1)
+    return set_error(NB_ERR_UNSUPPORTED, "no band plan for this size");
+  a.fpart = scratch;
+  a.spart = scratch + (size_t)bp.rows * R::ACC * ST;
+  a.acc3 = a.spart + (size_t)bp.reacs * R::ACC * ST;
+  a.ts = bp.ts;
+  a.chunk = chunk;
+  a.n = n;
+  void *args[] = {&a};
+  const dim3 block(256), part_grid((n + 255) / 256);
+  for (int b0 = 0; b0 < bp.ts; b0 += bp.w) {
+    a.b0 = b0;
+    a.bw = std::min(bp.w, bp.ts - b0);
+    a.first = b0 == 0;
+    const long long items = band_items(a.b0, a.bw, chunk);
+    int r = hip_check(
+        hipLaunchKernel(reinterpret_cast<const void *>(sym_band_kernel<R, U>),
+                        dim3((unsigned)items), block, args, 0, stream),
+        "symmetric band launch");
+    if (!r)
+      r = hip_check(
+          hipLaunchKernel(reinterpret_cast<const void *>(sym_band_fold<R, U>),
+                          part_grid, block, args, 0, stream),
+          "symmetric band fold launch");
+    if (r) return r;
+  }
+  return hip_check(
+      hipLaunchKernel(reinterpret_cast<const void *>(sym_band_finish<R, U>),
+                      part_grid, block, args, 0, stream),
+      "symmetric band finish launch");
+}
+
 int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
                float *scratch, int acc, bool tiles, hipStream_t stream) {
+  if (sym_banded(unroll, tiles, chunk, acc, (size_t)n) &&
+      sh == find_shape<ForceRule>(unroll, tiles)) {
+    if (!scratch)
+      return set_error(NB_ERR_INVALID_ARG, "symmetric launch needs scratch");
+    return launch_sym_banded(a, chunk, n, scratch, stream);
+  }
   const size_t cap = sym_max_particles(unroll, tiles, chunk, acc);
   if ((size_t)n > cap)
     return set_error(
@@ -1061,8 +1250,8 @@ int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
         tiles         ? "symmetric tile-pair shapes take at most 2^17 particles"
         : unroll <= 2 ? "symmetric shapes with 2 per lane take at most 2^18 particles"
         : cap == kSymLargeMaxParticles
-            ? "symmetric launch shapes take at most 2^21 particles"
-            : "symmetric launch shapes take at most 2^20 particles (2^21 with "
+            ? "symmetric launch shapes take at most 2^24 particles"
+            : "symmetric launch shapes take at most 2^20 particles (2^24 with "
               "plain sums, 8 per lane and 4 super-tiles per item)");
   if (!sh || chunk < 1)
     return set_error(NB_ERR_UNSUPPORTED, "no symmetric kernel for this shape");
@@ -1110,6 +1299,13 @@ size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc,
     const long long P = 64LL * unroll;
     const long long T = ((long long)n + P - 1) / P;
     return sizeof(float) * (size_t)(T * (T + 1) / 2) * 2 * acc * (size_t)P;
+  }
+  if (sym_banded(unroll, tiles, chunk, acc, n)) {
+    const BandPlan bp = band_plan(n, unroll, chunk, acc);
+    if (bp.w < 1) return 0;
+    return sizeof(float) * ((size_t)(bp.rows + bp.reacs) * acc *
+                                (size_t)(kWaves * 64 * unroll) +
+                            (size_t)acc * n);
   }
   if (n == 0 || n > sym_max_particles(unroll, tiles, chunk, acc) || chunk < 1)
     return 0;

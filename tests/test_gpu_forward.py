@@ -336,6 +336,28 @@ def test_two_million_sampled_step(nb, orc, gpu):
                            np.arange(i0, i1), f"N=2^21 targets [{i0},{i1})")
 
 
+def test_banded_step_past_two_million(nb, orc, gpu):
+    """N = 2^21 + 4097: the plain step runs in bands of source super-tiles
+    (two here, each with at most 8 GB of partials), summed band by band:
+    one step against the oracle on the first targets and the last, partial
+    super-tile."""
+    n = (1 << 21) + 4097
+    pos, vel = nb.disk_ic(n)
+    params = nb.default_params(num_particles=n, sim_iterations_per_frame=1)
+    sim = nb.Simulator(params, pos, vel)
+    info = nb.forward_config_info(sim.config)
+    assert (info["scheme"], info["unroll"], info["split"]) == ("symmetric", 8, 4), info
+    sim.step_sim()
+    gp = sim.positions()
+    sim.close()
+    p = orc.params(num_particles=n)
+    for (i0, i1) in [(0, 2048), (n - 2048, n)]:
+        rp, _ = orc.step_f32(p, pos, vel, i0, i1)
+        sl = slice(i0, i1)
+        assert_step_parity(orc, p, pos, vel, [a[sl] for a in gp], [a[sl] for a in rp],
+                           np.arange(i0, i1), f"N=2^21+4097 targets [{i0},{i1})")
+
+
 def test_reference_driver_runs(nb, gpu):
     """The reference's own src/nbody.cpp, compiled unchanged against this
     project's simulator.hpp (oracle/_ref/nbody_dropin), drives the GPU."""
@@ -516,7 +538,7 @@ def test_symmetric_scheme_rejects_other_modes(nb, gpu):
     sim.set_accumulation(nb.NB_ACC_PLAIN)
     assert nb.forward_config_info(sim.config)["scheme"] != "targets"
     sim.close()
-    # beyond 2^20 particles (2^17 for tile pairs, 2^18 with 2 per lane, 2^21
+    # beyond 2^20 particles (2^17 for tile pairs, 2^18 with 2 per lane, 2^24
     # with plain sums, 8 per lane and 4 per item): refused before any launch
     m = (1 << 20) + 1
     big = [torch.zeros(m, device=gpu) for _ in range(9)]
@@ -529,12 +551,12 @@ def test_symmetric_scheme_rejects_other_modes(nb, gpu):
     with pytest.raises(nb.NbError, match="2\\^17"):
         nb.forward_step(nb.default_params(num_particles=m), [t[:m] for t in big[:3]],
                         [t[:m] for t in big[3:6]], [t[:m] for t in big[6:]], cfg=tiles)
-    large = next(c for c in sym_configs(nb)  # 8 per lane, 4 per item: 2^21
+    large = next(c for c in sym_configs(nb)  # 8 per lane, 4 per item: banded to 2^24
                  if nb.forward_config_info(c)["unroll"] == 8
                  and nb.forward_config_info(c)["split"] >= 4)
-    m2 = (1 << 21) + 1
+    m2 = (1 << 24) + 1
     big2 = [torch.zeros(m2, device=gpu) for _ in range(9)]
-    with pytest.raises(nb.NbError, match="2\\^21"):
+    with pytest.raises(nb.NbError, match="2\\^24"):
         nb.forward_step(nb.default_params(num_particles=m2), big2[:3], big2[3:6], big2[6:],
                         cfg=large)
     m20 = (1 << 20) + 1
