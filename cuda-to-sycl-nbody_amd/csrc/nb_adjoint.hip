@@ -298,6 +298,7 @@ const FwdConfig kAdjConfigs[] = {
     {256, 4, 256, 1, SCHEME_SYMMETRIC_TILES},
     {256, 2, 128, 1, SCHEME_SYMMETRIC_TILES},
     {256, 2, 512, 1, SCHEME_SYMMETRIC}, {256, 2, 512, 2, SCHEME_SYMMETRIC},
+    {256, 4, 1024, 4, SCHEME_SYMMETRIC},
 };
 constexpr int kNumAdj = sizeof(kAdjConfigs) / sizeof(kAdjConfigs[0]);
 
@@ -322,6 +323,8 @@ int adjoint_default_config(size_t n, bool skip) {
   // 4780 / 6511 vs 4779 / 6478, 262144 19280 / 25892 vs 19168 / 25601
   // (4 per lane 23660 / 31135, 8 per lane 24283 / 34040).  Best
   // target-sharded: 8192 65.8 / 74.2, 262144 38.7 / 43.5 ms.
+  if (n <= kSymBandedMaxParticles && n > kSymMaxParticles)
+    return 21; // super-tile pairs, 4 per lane, 4 per item, in bands
   if (n <= kSymMaxParticles) { // skip or not: self pairs masked in-kernel
     if (n > kSymSmallMaxParticles) return 14; // super-tile pairs, 4 per lane
     if (n >= 98304) return 20; // super-tile pairs, 2 per lane, 2 per item
@@ -335,9 +338,12 @@ int adjoint_default_config(size_t n, bool skip) {
 
 size_t adjoint_scratch_bytes(int cfg, size_t n) {
   const FwdConfig &c = kAdjConfigs[cfg];
-  if (c.scheme != SCHEME_TARGETS) // sized for the fused variant
-    return sym_scratch_bytes(c.unroll, c.split, n, 6,
-                             c.scheme == SCHEME_SYMMETRIC_TILES);
+  if (c.scheme != SCHEME_TARGETS) { // either variant (band widths differ)
+    const bool tiles = c.scheme == SCHEME_SYMMETRIC_TILES;
+    const size_t fused = sym_scratch_bytes(c.unroll, c.split, n, 6, tiles, true);
+    const size_t plain = sym_scratch_bytes(c.unroll, c.split, n, 3, tiles, true);
+    return fused && plain ? std::max(fused, plain) : 0;
+  }
   return c.split > 1 ? sizeof(float) * 6 * (size_t)c.split * n : 0;
 }
 

@@ -182,7 +182,8 @@ int forward_default_config(size_t n, int mode) {
   // super-tiles of 8 per lane per item run as fast as one and need 25 %
   // less scratch (6042 Gint/s at 2^20); past 2^20 four per item keep the
   // partials at 16 GB for 2^21 particles, and past 2^21 the step runs in
-  // bands of source super-tiles with at most 8 GB of partials each.
+  // bands of source super-tiles with at most 8 GB of partials each (the
+  // 4-per-lane shape with 4 per item is banded past 2^20).
   if (mode == MODE_PLAIN && n <= kSymBandedMaxParticles) {
     if (n > kSymMaxParticles) return 24; // 8 per lane, 4 per item (banded past 2^21)
     if (n > ((size_t)1 << 19)) return 23; // symmetric, 8 per lane, 2 per item
@@ -220,10 +221,11 @@ int forward_default_config(size_t n, int mode) {
 size_t forward_scratch_bytes(int cfg, size_t n, int mode) {
   const FwdConfig &c = kConfigs[cfg];
   if (c.scheme != SCHEME_TARGETS) // (sum, compensation) pairs for Kahan
-    return sym_scratch_bytes(
-        c.unroll, c.split, n,
-        mode == MODE_KAHAN || mode == MODE_KAHAN_SKIP ? 6 : 3,
-        c.scheme == SCHEME_SYMMETRIC_TILES);
+  {
+    const bool kahan = mode == MODE_KAHAN || mode == MODE_KAHAN_SKIP;
+    return sym_scratch_bytes(c.unroll, c.split, n, kahan ? 6 : 3,
+                             c.scheme == SCHEME_SYMMETRIC_TILES, !kahan);
+  }
   return c.split > 1 ? sizeof(float) * 3 * (size_t)c.split * n : 0;
 }
 

@@ -80,9 +80,14 @@ constexpr size_t kSymSmallMaxParticles = (size_t)1 << 18;
 // and at least four super-tiles per work item (16 GB of partials at 2^21).
 constexpr size_t kSymLargeMaxParticles = (size_t)1 << 21;
 
-// Past kSymLargeMaxParticles the plain forward shapes with 8 per lane and at
-// least four super-tiles per item run in bands of source super-tiles whose
-// partials take at most kSymBandBytes, up to kSymBandedMaxParticles.
+// Past their single-launch limit (2^21 for 8 per lane with plain sums, 2^20
+// otherwise) the shapes with 4 or 8 per lane and at least four super-tiles
+// per item run the plain forward step and both adjoint steps in bands of
+// source super-tiles, up to kSymBandedMaxParticles.  A band's partials take
+// at most kSymBandBytes at 3 sums per particle (twice that for the fused
+// adjoint's 6): the band width depends on the slot count only, so the fused
+// adjoint cuts the bands the forward step cuts and its forward part keeps
+// the forward step's bits.
 constexpr size_t kSymBandedMaxParticles = (size_t)1 << 24;
 constexpr size_t kSymBandBytes = (size_t)8 << 30;
 
@@ -97,10 +102,13 @@ constexpr size_t sym_max_particles(int unroll, bool tiles, int chunk = 1,
 }
 
 // Symmetric scheme (nb_sym.hip; BRANCH, any eps): scratch bytes
-// for `acc` sums per particle (3 forward and adjoint, 6 fused adjoint) and
-// stream-ordered launches of one forward step / one adjoint step.
+// for `acc` sums per particle (3 forward and adjoint, 6 fused adjoint or
+// Kahan forward) and stream-ordered launches of one forward step / one
+// adjoint step.  `bandable`: the rule has banded kernels (plain forward and
+// both adjoints), which take super-tile shapes with 4 or 8 per lane and at
+// least 4 super-tiles per item past their single-launch limit.
 size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc,
-                         bool tiles);
+                         bool tiles, bool bandable);
 int launch_adjoint_sym(const StepConsts &c, bool fused, int unroll, int chunk,
                        bool tiles, const float *const pos[3], const float *const lxn[3],
                        const float *const lvn[3], float *const lx[3],

@@ -843,6 +843,7 @@ __global__ __launch_bounds__(256) void sym_band_fold(SymArgs a) {
   if (i >= a.n) return;
   const int g = i / ST, o = i % ST;
   const int L = a.chunk, b0 = a.b0, W = a.bw, b1 = b0 + W;
+  if (g >= b1 && !a.first) return; // no band term; its sum stays
   float f[R::ACC];
 #pragma unroll
   for (int e = 0; e < R::ACC; ++e) f[e] = 0.0f;
@@ -1166,24 +1167,43 @@ const SymShape *find_kahan_shape(int unroll, bool tiles) {
   return nullptr;
 }
 
-// Banded plain forward step: the shape (8 per lane, >= 4 super-tiles per
-// item) and size range it serves.
+// Banded launches: super-tile shapes with 4 or 8 particles per lane and at
+// least four super-tiles per item, past their single-launch limit.
 bool sym_banded(int unroll, bool tiles, int chunk, int acc, size_t n) {
-  return !tiles && unroll == 8 && chunk >= 4 && acc == 3 &&
-         n > kSymLargeMaxParticles && n <= kSymBandedMaxParticles;
+  return !tiles && (unroll == 4 || unroll == 8) && chunk >= 4 &&
+         n > sym_max_particles(unroll, tiles, chunk, acc) &&
+         n <= kSymBandedMaxParticles;
 }
 
-// Band width (the widest whose partials fit kSymBandBytes) and the largest
-// row-item and reaction slot counts over the bands.
+struct BandShape {
+  int unroll;
+  KernelFn pairs, fold, finish;
+};
+
+// Band kernels of rule R (plain force and adjoint rules; none for Kahan).
+template <class R>
+const BandShape *find_band_shape(int unroll) {
+  static const BandShape shapes[] = {
+      {4, sym_band_kernel<R, 4>, sym_band_fold<R, 4>, sym_band_finish<R, 4>},
+      {8, sym_band_kernel<R, 8>, sym_band_fold<R, 8>, sym_band_finish<R, 8>},
+  };
+  for (const BandShape &s : shapes)
+    if (s.unroll == unroll) return &s;
+  return nullptr;
+}
+
+// Band width (the widest whose partials fit kSymBandBytes at 3 sums per
+// particle, whatever the rule's count) and the largest row-item and reaction
+// slot counts over the bands.
 struct BandPlan {
   int ts, w;
   long long rows, reacs;
 };
 
-BandPlan band_plan(size_t n, int unroll, int chunk, int acc) {
+BandPlan band_plan(size_t n, int unroll, int chunk) {
   const int st = kWaves * 64 * unroll;
   const int ts = (int)((n + st - 1) / st);
-  const size_t per_slot = sizeof(float) * acc * st;
+  const size_t per_slot = sizeof(float) * 3 * st;
   for (int w = ts; w >= 1; --w) {
     long long rows = 0, reacs = 0;
     for (int b0 = 0; b0 < ts; b0 += w) {
@@ -1197,17 +1217,16 @@ BandPlan band_plan(size_t n, int unroll, int chunk, int acc) {
   return {ts, 0, 0, 0};
 }
 
-int launch_sym_banded(SymArgs &a, int chunk, int n, float *scratch,
-                      hipStream_t stream) {
-  using R = ForceRule;
-  constexpr int U = 8;
-  constexpr int ST = kWaves * 64 * U;
-  const BandPlan bp = band_plan((size_t)n, U, chunk, R::ACC);
+int launch_sym_banded(const BandShape *band, SymArgs &a, int chunk, int n,
+                      float *scratch, int acc, hipStream_t stream) {
+  const int unroll = band->unroll;
+  const int st = kWaves * 64 * unroll;
+  const BandPlan bp = band_plan((size_t)n, unroll, chunk);
   if (bp.w < 1)
     return set_error(NB_ERR_UNSUPPORTED, "no band plan for this size");
   a.fpart = scratch;
-  a.spart = scratch + (size_t)bp.rows * R::ACC * ST;
-  a.acc3 = a.spart + (size_t)bp.reacs * R::ACC * ST;
+  a.spart = scratch + (size_t)bp.rows * acc * st;
+  a.acc3 = a.spart + (size_t)bp.reacs * acc * st;
   a.ts = bp.ts;
   a.chunk = chunk;
   a.n = n;
@@ -1219,29 +1238,28 @@ int launch_sym_banded(SymArgs &a, int chunk, int n, float *scratch,
     a.first = b0 == 0;
     const long long items = band_items(a.b0, a.bw, chunk);
     int r = hip_check(
-        hipLaunchKernel(reinterpret_cast<const void *>(sym_band_kernel<R, U>),
+        hipLaunchKernel(reinterpret_cast<const void *>(band->pairs),
                         dim3((unsigned)items), block, args, 0, stream),
         "symmetric band launch");
     if (!r)
-      r = hip_check(
-          hipLaunchKernel(reinterpret_cast<const void *>(sym_band_fold<R, U>),
-                          part_grid, block, args, 0, stream),
-          "symmetric band fold launch");
+      r = hip_check(hipLaunchKernel(reinterpret_cast<const void *>(band->fold),
+                                    part_grid, block, args, 0, stream),
+                    "symmetric band fold launch");
     if (r) return r;
   }
   return hip_check(
-      hipLaunchKernel(reinterpret_cast<const void *>(sym_band_finish<R, U>),
-                      part_grid, block, args, 0, stream),
+      hipLaunchKernel(reinterpret_cast<const void *>(band->finish), part_grid,
+                      block, args, 0, stream),
       "symmetric band finish launch");
 }
 
 int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
-               float *scratch, int acc, bool tiles, hipStream_t stream) {
-  if (sym_banded(unroll, tiles, chunk, acc, (size_t)n) &&
-      sh == find_shape<ForceRule>(unroll, tiles)) {
+               float *scratch, int acc, bool tiles, hipStream_t stream,
+               const BandShape *band = nullptr) {
+  if (band && sym_banded(unroll, tiles, chunk, acc, (size_t)n)) {
     if (!scratch)
       return set_error(NB_ERR_INVALID_ARG, "symmetric launch needs scratch");
-    return launch_sym_banded(a, chunk, n, scratch, stream);
+    return launch_sym_banded(band, a, chunk, n, scratch, acc, stream);
   }
   const size_t cap = sym_max_particles(unroll, tiles, chunk, acc);
   if ((size_t)n > cap)
@@ -1249,10 +1267,10 @@ int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
         NB_ERR_UNSUPPORTED,
         tiles         ? "symmetric tile-pair shapes take at most 2^17 particles"
         : unroll <= 2 ? "symmetric shapes with 2 per lane take at most 2^18 particles"
-        : cap == kSymLargeMaxParticles
+        : chunk >= 4 && band
             ? "symmetric launch shapes take at most 2^24 particles"
             : "symmetric launch shapes take at most 2^20 particles (2^24 with "
-              "plain sums, 8 per lane and 4 super-tiles per item)");
+              "4 or 8 per lane and 4 super-tiles per item, plain sums)");
   if (!sh || chunk < 1)
     return set_error(NB_ERR_UNSUPPORTED, "no symmetric kernel for this shape");
   if (!scratch)
@@ -1293,15 +1311,15 @@ int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
 } // namespace
 
 size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc,
-                         bool tiles) {
+                         bool tiles, bool bandable) {
   if (tiles) {
     if (n == 0 || n > sym_max_particles(unroll, tiles, chunk, acc)) return 0;
     const long long P = 64LL * unroll;
     const long long T = ((long long)n + P - 1) / P;
     return sizeof(float) * (size_t)(T * (T + 1) / 2) * 2 * acc * (size_t)P;
   }
-  if (sym_banded(unroll, tiles, chunk, acc, n)) {
-    const BandPlan bp = band_plan(n, unroll, chunk, acc);
+  if (bandable && sym_banded(unroll, tiles, chunk, acc, n)) {
+    const BandPlan bp = band_plan(n, unroll, chunk);
     if (bp.w < 1) return 0;
     return sizeof(float) * ((size_t)(bp.rows + bp.reacs) * acc *
                                 (size_t)(kWaves * 64 * unroll) +
@@ -1422,7 +1440,8 @@ int launch_forward_sym(const StepConsts &c, int unroll, int chunk, bool tiles,
     return launch_sym(find_kahan_shape(unroll, tiles), a, unroll, chunk, n,
                       scratch, KahanForceRule::ACC, tiles, stream);
   return launch_sym(find_shape<ForceRule>(unroll, tiles), a, unroll, chunk, n,
-                    scratch, ForceRule::ACC, tiles, stream);
+                    scratch, ForceRule::ACC, tiles, stream,
+                    tiles ? nullptr : find_band_shape<ForceRule>(unroll));
 }
 
 int launch_adjoint_sym(const StepConsts &c, bool fused, int unroll, int chunk,
@@ -1450,11 +1469,14 @@ int launch_adjoint_sym(const StepConsts &c, bool fused, int unroll, int chunk,
     a.nx = nxt[0];
     a.ny = nxt[1];
     a.nz = nxt[2];
-    return launch_sym(find_shape<AdjointRule<true>>(unroll, tiles), a, unroll,
-                      chunk, n, scratch, AdjointRule<true>::ACC, tiles, stream);
+    return launch_sym(
+        find_shape<AdjointRule<true>>(unroll, tiles), a, unroll, chunk, n,
+        scratch, AdjointRule<true>::ACC, tiles, stream,
+        tiles ? nullptr : find_band_shape<AdjointRule<true>>(unroll));
   }
   return launch_sym(find_shape<AdjointRule<false>>(unroll, tiles), a, unroll,
-                    chunk, n, scratch, AdjointRule<false>::ACC, tiles, stream);
+                    chunk, n, scratch, AdjointRule<false>::ACC, tiles, stream,
+                    tiles ? nullptr : find_band_shape<AdjointRule<false>>(unroll));
 }
 
 } // namespace nb

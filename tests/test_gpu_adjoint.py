@@ -176,6 +176,82 @@ def fused_equals_separate(nb, gpu, n, acfg):
     assert d < 1e-6, (acfg, float(d))
 
 
+def test_banded_adjoint_past_one_million(nb, orc, gpu):
+    """N = 1310720 + 777 (1281 super-tiles of 1024, cut into two bands of
+    source super-tiles): the library's adjoint shape (4 per lane, 4 per item)
+    and its forward twin run in bands.  With dt = G = 1 and lam_x' = 0,
+    lam_x is the pair sum J^T g: checked against the target-sharded adjoint
+    (itself checked against the fp64 oracle at 3000) and by its total,
+    sum_k sum_j M_kj (g_j - g_k) = 0, which a lost or mis-signed reaction
+    breaks.  The fused step: forward part bit-equal to the banded forward
+    step, whose forces net to zero and are as close to the oracle's fp64
+    forces as the target-sharded step's."""
+    import torch
+
+    n = 1310720 + 777
+    band = next(c for c in sym_adjoint_configs(nb)
+                if (nb.adjoint_config_info(c)["scheme"], nb.adjoint_config_info(c)["unroll"],
+                    nb.adjoint_config_info(c)["split"]) == ("symmetric", 4, 4))
+    targets = next(c for c in range(nb.adjoint_config_count())
+                   if nb.adjoint_config_info(c)["scheme"] == "targets"
+                   and nb.adjoint_config_info(c)["split"] > 1)
+    ftargets = next(c for c in range(nb.forward_config_count())
+                    if nb.forward_config_info(c) == nb.adjoint_config_info(targets))
+    pos, _ = nb.disk_ic(n)
+    params = nb.default_params(num_particles=n, dt=1.0, G=1.0, damping=0.5)
+    rng = np.random.default_rng(7)
+    P = to_dev(torch, pos, gpu)
+    lxn = torch.zeros(3, n, device=gpu)
+    lvn = torch.from_numpy(rng.normal(size=(3, n)).astype(np.float32)).to(gpu)
+    out = {}
+    for name, cfg in (("default", -1), ("band", band), ("targets", targets)):
+        lx, lv = torch.empty(3, n, device=gpu), torch.empty(3, n, device=gpu)
+        print(f"adjoint N={n} {name} cfg {cfg}", flush=True)
+        nb.adjoint_step(params, P, lxn, lvn, lx, lv, cfg=cfg)
+        torch.cuda.synchronize()
+        out[name] = lx
+    assert torch.equal(out["default"], out["band"])
+    got, ref = out["band"].double(), out["targets"].double()
+    e = float((got - ref).norm() / ref.norm())
+    scale = got.abs().sum(1)
+    net = (got.sum(1).abs() / scale).cpu().numpy()
+    net_t = (ref.sum(1).abs() / scale).cpu().numpy()
+    print(f"banded vs targets {e:.2e}; net / sum|.| banded {net}, targets {net_t}")
+    assert e < 1e-5
+    assert np.all(net <= 1e-5) and np.all(net <= 2 * net_t + 1e-7)
+    # fused step and the forward steps (zero velocities: vel' = dt G F)
+    fcfg = matching_forward_config(nb, band)
+    V = {k: torch.zeros(3, n, device=gpu) for k in ("fwd", "fused", "targets")}
+    N = {k: torch.empty(3, n, device=gpu) for k in V}
+    print(f"forward N={n} cfg {fcfg} / {ftargets}, fused cfg {band}", flush=True)
+    nb.forward_step(params, list(P), list(V["fwd"]), list(N["fwd"]), cfg=fcfg)
+    nb.forward_step(params, list(P), list(V["targets"]), list(N["targets"]), cfg=ftargets)
+    lx2, lv2 = torch.empty(3, n, device=gpu), torch.empty(3, n, device=gpu)
+    nb.fwd_adjoint_step(params, P, V["fused"], N["fused"], lxn, lvn, lx2, lv2, cfg=band)
+    torch.cuda.synchronize()
+    assert torch.equal(V["fwd"], V["fused"]) and torch.equal(N["fwd"], N["fused"])
+    d = float((lx2.double() - got).norm() / got.norm())
+    assert d < 1e-6, d
+    # the disk's net forces cancel deeply (fp32 sums differ by 1e-3 of |F|
+    # for some particles): both schemes against fp64 forces on 512 targets,
+    # the first and the last (partial) super-tile's
+    F = V["fwd"].double()
+    fscale = F.abs().sum(1)
+    fnet = (F.sum(1).abs() / fscale).cpu().numpy()
+    idx = np.concatenate([np.arange(256), np.arange(n - 256, n)])
+    f64 = np.stack(orc.forces_f64(pos, idx))
+    err = {}
+    for k in ("fwd", "targets"):
+        g = V[k].cpu().numpy()[:, idx].astype(np.float64)
+        err[k] = np.linalg.norm(g - f64, axis=0) / np.linalg.norm(f64, axis=0)
+    print(f"force err vs fp64: banded p50 {np.median(err['fwd']):.2e} max "
+          f"{err['fwd'].max():.2e}; targets p50 {np.median(err['targets']):.2e} max "
+          f"{err['targets'].max():.2e}; net force / sum|F| {fnet}; fused lx {d:.2e}")
+    assert np.all(fnet <= 1e-5)
+    assert np.median(err["fwd"]) <= 2 * np.median(err["targets"]) + 1e-7
+    assert err["fwd"].max() <= 2 * err["targets"].max() + 1e-6
+
+
 def test_every_adjoint_shape(nb, gpu):
     import torch
 

@@ -539,7 +539,8 @@ def test_symmetric_scheme_rejects_other_modes(nb, gpu):
     assert nb.forward_config_info(sim.config)["scheme"] != "targets"
     sim.close()
     # beyond 2^20 particles (2^17 for tile pairs, 2^18 with 2 per lane, 2^24
-    # with plain sums, 8 per lane and 4 per item): refused before any launch
+    # with plain sums, 4 or 8 per lane and 4 per item): refused before any
+    # launch
     m = (1 << 20) + 1
     big = [torch.zeros(m, device=gpu) for _ in range(9)]
     with pytest.raises(nb.NbError, match="2\\^20"):
