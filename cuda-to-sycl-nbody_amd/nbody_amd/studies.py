@@ -220,11 +220,12 @@ def adjoint(n: int = 16384, steps: int = 200, every: int = 20, fd_coords: int = 
     return out
 
 
-def shapes(sizes, kinds=("forward", "adjoint", "kahan")) -> dict:
-    """Times every compiled launch shape on the reference disk at each size:
-    the forward step (plain and Kahan accumulation) and the adjoint (plain
-    and fused with the forward step).  Shapes that refuse a size or mode are
-    skipped.  Returns {kind: {n: [(cfg, info, ms), ...]}}, fastest first."""
+def shapes(sizes, kinds=("forward", "adjoint", "kahan"), cfgs=None) -> dict:
+    """Times every compiled launch shape (or those listed in cfgs) on the
+    reference disk at each size: the forward step (plain and Kahan
+    accumulation) and the adjoint (plain and fused with the forward step).
+    Shapes that refuse a size or mode are skipped.  Returns
+    {kind: {n: [(cfg, info, ms), ...]}}, fastest first."""
     import nbody_amd as nb
 
     torch = _torch()
@@ -255,6 +256,8 @@ def shapes(sizes, kinds=("forward", "adjoint", "kahan")) -> dict:
             if kind in ("forward", "kahan"):
                 acc = nb.NB_ACC_KAHAN if kind == "kahan" else nb.NB_ACC_PLAIN
                 for cfg in range(nb.forward_config_count()):
+                    if cfgs is not None and cfg not in cfgs:
+                        continue
                     try:
                         ms = timed(lambda: nb.forward_step(params, P, V, N, cfg=cfg, acc=acc),
                                    reps)
@@ -270,6 +273,8 @@ def shapes(sizes, kinds=("forward", "adjoint", "kahan")) -> dict:
                 V3 = torch.stack(V)
                 N3 = torch.empty_like(P3)
                 for cfg in range(nb.adjoint_config_count()):
+                    if cfgs is not None and cfg not in cfgs:
+                        continue
                     try:
                         ms = timed(lambda: nb.adjoint_step(params, P3, lxn, lvn, lx, lv,
                                                            cfg=cfg), reps)
@@ -292,6 +297,9 @@ def main(argv=None) -> int:
     ap.add_argument("study", choices=["precision", "adjoint", "shapes"])
     ap.add_argument("--sizes", type=int, nargs="+", default=[16384, 65536, 262144],
                     help="shapes: particle counts")
+    ap.add_argument("--kinds", nargs="+", choices=["forward", "adjoint", "kahan"],
+                    default=["forward", "adjoint", "kahan"], help="shapes: what to time")
+    ap.add_argument("--cfgs", type=int, nargs="+", help="shapes: only these shape indices")
     ap.add_argument("--n", type=int)
     ap.add_argument("--steps", type=int)
     ap.add_argument("--every", type=int, default=20)
@@ -303,7 +311,7 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     _lib.load()
     if a.study == "shapes":
-        res = shapes(a.sizes)
+        res = shapes(a.sizes, tuple(a.kinds), a.cfgs)
     elif a.study == "precision":
         res = precision(n=a.n or 1048576, steps=a.steps or 100, plummer_a=a.plummer_a,
                         damping=a.damping, eps=a.eps)
