@@ -111,6 +111,59 @@ def test_adjoint_vanishing_eps(nb, orc, gpu):
             assert rel(got, ref) < 1e-5, (what, rel(got, ref))
 
 
+@pytest.mark.parametrize("eps", ["default", 0.0])
+def test_adjoint_coincident_particles(nb, orc, gpu, eps):
+    """Distinct particles at one position (3001 particles; pairs within a
+    tile, across tiles, in the last partial tile, and a triple), every
+    adjoint shape, plain and fused.  With SimParam's eps a coincident pair
+    adds K (g_j - g_k), K = eps^-3/2: the pair sum matches the fp64 adjoint.
+    With eps = 0 the coincident pairs give NaN (d = 0), for exactly those
+    particles, in the fp64 adjoint and in every shape; the others match."""
+    import torch
+
+    n = 3001
+    pos, vel = nb.disk_ic(n)
+    pos = [a.copy() for a in pos]
+    for dst, src in ((1, 0), (1500, 5), (3000, 2999), (2990, 100), (701, 700), (2000, 700)):
+        for a in pos:
+            a[dst] = a[src]
+    kw = {} if eps == "default" else {"dist_eps": eps}
+    rng = np.random.default_rng(11)
+    g = rng.normal(size=(3, n)).astype(np.float32)
+    params = nb.default_params(num_particles=n, dt=1.0, G=1.0, damping=0.5, **kw)
+    p = orc.params(dt=1.0, G=1.0, damping=0.5, **kw)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ref_lx, _ = orc.adjoint_step_f64(p, pos, np.zeros((3, n)), g.astype(np.float64))
+    ref = np.stack(ref_lx)
+    ref_nan = ~np.isfinite(ref).all(0)
+    dup = np.zeros(n, bool)
+    dup[[1, 0, 1500, 5, 3000, 2999, 2990, 100, 701, 700, 2000]] = True
+    assert np.array_equal(ref_nan, dup if eps == 0.0 else np.zeros(n, bool))
+    # the coincident pairs' K = eps^-3/2 dominates their particles' sums:
+    # those and the others are compared separately
+    groups = [~dup] if eps == 0.0 else [~dup, dup]
+    P = to_dev(torch, pos, gpu)
+    V = to_dev(torch, vel, gpu)
+    lxn = torch.zeros(3, n, device=gpu)
+    lvn = torch.from_numpy(g).to(gpu)
+    for cfg in range(nb.adjoint_config_count()):
+        for fused in (False, True):
+            lx = torch.empty(3, n, device=gpu)
+            lv = torch.empty(3, n, device=gpu)
+            if fused:
+                nb.fwd_adjoint_step(params, P, V.clone(), torch.empty_like(P), lxn, lvn,
+                                    lx, lv, cfg=cfg)
+            else:
+                nb.adjoint_step(params, P, lxn, lvn, lx, lv, cfg=cfg)
+            torch.cuda.synchronize()
+            got = lx.cpu().numpy().astype(np.float64)
+            what = (eps, cfg, fused, nb.adjoint_config_info(cfg))
+            assert np.array_equal(~np.isfinite(got).all(0), ref_nan), what
+            for sel in groups:
+                e = rel(got[:, sel], ref[:, sel])
+                assert e < 1e-5, (what, e)
+
+
 @pytest.mark.parametrize("n", [4096, 33000])
 def test_adjoint_step_default_params(nb, orc, gpu, n):
     """SimParam defaults, the library's shape (tile pairs at 4096, super-tile

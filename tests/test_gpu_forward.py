@@ -467,6 +467,54 @@ def test_symmetric_scheme_vanishing_eps(nb, orc, gpu, eps, n):
             assert_step_parity(orc, p, pos, vel, got, rp, np.arange(n), what)
 
 
+COINCIDENT = [(1, 0), (1500, 5), (3000, 2999), (2990, 100), (701, 700), (2000, 700)]
+
+
+@pytest.mark.parametrize("eps", ["default", 0.0])
+def test_coincident_particles(nb, orc, gpu, eps):
+    """Distinct particles at one position (3001 particles; pairs within a
+    tile, across tiles, in the last partial tile, and a triple).  With
+    SimParam's eps the pair term r K is an exact zero: every launch shape,
+    plain and Kahan sums, in parity with the oracle step.  With eps = 0 the
+    reference's arithmetic gives 0 * rsqrt(0) = NaN for exactly those
+    particles (its `continue` skips only j == i): every shape gives NaN for
+    the same particles as the oracle and parity for all others."""
+    import torch
+
+    n = 3001
+    pos, vel = nb.disk_ic(n)
+    pos = [a.copy() for a in pos]
+    for dst, src in COINCIDENT:
+        for a in pos:
+            a[dst] = a[src]
+    kw = {} if eps == "default" else {"dist_eps": eps}
+    params = nb.default_params(num_particles=n, **kw)
+    p = orc.params(num_particles=n, **kw)
+    rp, _ = orc.step_f32(p, pos, vel)
+    dup = sorted({i for pr in COINCIDENT for i in pr})
+    ref_nan = ~np.isfinite(np.stack(rp)).all(0)
+    if eps == "default":
+        assert not ref_nan.any()
+    else:
+        assert np.array_equal(np.nonzero(ref_nan)[0], dup)
+    keep = np.nonzero(~ref_nan)[0]
+    for cfg in range(nb.forward_config_count()):
+        info = nb.forward_config_info(cfg)
+        accs = [nb.NB_ACC_PLAIN]
+        if info["scheme"] == "targets" or info["unroll"] <= 4:
+            accs.append(nb.NB_ACC_KAHAN)
+        for acc in accs:
+            dp, dv = dev3(torch, pos, gpu), dev3(torch, vel, gpu)
+            dn = [torch.empty_like(t) for t in dp]
+            nb.forward_step(params, dp, dv, dn, cfg=cfg, acc=acc)
+            torch.cuda.synchronize()
+            got = [t.cpu().numpy() for t in dn]
+            what = f"eps={eps} cfg {cfg} {info} acc {acc}"
+            assert np.array_equal(~np.isfinite(np.stack(got)).all(0), ref_nan), what
+            assert_step_parity(orc, p, pos, vel, [a[keep] for a in got],
+                               [a[keep] for a in rp], keep, what)
+
+
 def test_symmetric_scheme_is_deterministic(nb, gpu):
     """Partials are summed in a fixed order: repeated steps are bit-identical."""
     import torch
