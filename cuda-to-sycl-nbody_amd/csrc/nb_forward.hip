@@ -151,6 +151,7 @@ const FwdConfig kConfigs[] = {
     {256, 4, 256, 1, SCHEME_SYMMETRIC_TILES},
     {256, 2, 128, 1, SCHEME_SYMMETRIC_TILES},
     {256, 2, 512, 1, SCHEME_SYMMETRIC}, {256, 2, 512, 2, SCHEME_SYMMETRIC},
+    {256, 2, 512, 4, SCHEME_SYMMETRIC},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
@@ -197,10 +198,12 @@ int forward_default_config(size_t n, int mode) {
   // 1 / 2 per item: 65536 2774 / 2674, 131072 2873 / 2867, 262144 2916 /
   // 2929 (4 per lane 2411, target-sharded 2373).  The compensated sums'
   // dependency chains bound this mode, so the shapes with the most waves
-  // win; below 12288 the target-sharded ones (8192: 1229 vs 1128), above
-  // 2^18 the symmetric partials ((sum, compensation) pairs) would need
-  // gigabytes of scratch.
+  // win; below 12288 the target-sharded ones (8192: 1229 vs 1128).  Above
+  // 2^18 four super-tiles per item run in bands of source super-tiles (16 GB
+  // of (sum, compensation) partials per band).
   if (mode == MODE_KAHAN) {
+    if (n > ((size_t)1 << 18) && n <= kSymBandedMaxParticles)
+      return 36;               // symmetric, 2 per lane, 4 per item, banded
     if (n >= ((size_t)1 << 17) && n <= ((size_t)1 << 18))
       return 35;               // symmetric, 2 per lane, 2 per item
     if (n >= 49152 && n < ((size_t)1 << 17))
@@ -224,7 +227,8 @@ size_t forward_scratch_bytes(int cfg, size_t n, int mode) {
   {
     const bool kahan = mode == MODE_KAHAN || mode == MODE_KAHAN_SKIP;
     return sym_scratch_bytes(c.unroll, c.split, n, kahan ? 6 : 3,
-                             c.scheme == SCHEME_SYMMETRIC_TILES, !kahan);
+                             c.scheme == SCHEME_SYMMETRIC_TILES,
+                             !kahan || c.unroll <= 4);
   }
   return c.split > 1 ? sizeof(float) * 3 * (size_t)c.split * n : 0;
 }

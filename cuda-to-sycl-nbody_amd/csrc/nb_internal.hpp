@@ -80,10 +80,11 @@ constexpr size_t kSymSmallMaxParticles = (size_t)1 << 18;
 // and at least four super-tiles per work item (16 GB of partials at 2^21).
 constexpr size_t kSymLargeMaxParticles = (size_t)1 << 21;
 
-// Past their single-launch limit (2^21 for 8 per lane with plain sums, 2^20
-// otherwise) the shapes with 4 or 8 per lane and at least four super-tiles
-// per item run the plain forward step and both adjoint steps in bands of
-// source super-tiles, up to kSymBandedMaxParticles.  A band's partials take
+// Past their single-launch limit (2^18 for 2 per lane, 2^21 for 8 per lane
+// with plain sums, 2^20 otherwise) the super-tile shapes with at least four
+// super-tiles per item run the forward step (plain or Kahan sums) and both
+// adjoint steps in bands of source super-tiles, up to
+// kSymBandedMaxParticles.  A band's partials take
 // at most kSymBandBytes at 3 sums per particle (twice that for the fused
 // adjoint's 6): the band width depends on the slot count only, so the fused
 // adjoint cuts the bands the forward step cuts and its forward part keeps
@@ -104,9 +105,10 @@ constexpr size_t sym_max_particles(int unroll, bool tiles, int chunk = 1,
 // Symmetric scheme (nb_sym.hip; BRANCH, any eps): scratch bytes
 // for `acc` sums per particle (3 forward and adjoint, 6 fused adjoint or
 // Kahan forward) and stream-ordered launches of one forward step / one
-// adjoint step.  `bandable`: the rule has banded kernels (plain forward and
-// both adjoints), which take super-tile shapes with 4 or 8 per lane and at
-// least 4 super-tiles per item past their single-launch limit.
+// adjoint step.  `bandable`: the rule has banded kernels for the shape
+// (every rule with 2, 4 or 8 per lane, Kahan with 2 or 4), which take the
+// super-tile shapes with at least 4 super-tiles per item past their
+// single-launch limit.
 size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc,
                          bool tiles, bool bandable);
 int launch_adjoint_sym(const StepConsts &c, bool fused, int unroll, int chunk,

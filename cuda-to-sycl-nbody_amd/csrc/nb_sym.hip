@@ -835,7 +835,8 @@ __global__ __launch_bounds__(256) void sym_band_kernel(SymArgs a) {
 }
 
 // Per particle: its band rows (chunk order) minus its band reactions (GA
-// order), added to the running sum.
+// order), added to the running sum (band order; Kahan sums carry their
+// compensation from band to band).
 template <class R, int U>
 __global__ __launch_bounds__(256) void sym_band_fold(SymArgs a) {
   constexpr int ST = kWaves * 64 * U;
@@ -862,11 +863,16 @@ __global__ __launch_bounds__(256) void sym_band_fold(SymArgs a) {
         },
         ST);
   }
+  float *acc = a.acc3 + i;
+  if (!a.first) {
+    float r[R::ACC];
+    R::fetch(r, acc, a.n);
+    R::add_v(r, f);
 #pragma unroll
-  for (int e = 0; e < R::ACC; ++e) {
-    float *acc = a.acc3 + (size_t)e * a.n + i;
-    *acc = a.first ? f[e] : *acc + f[e];
+    for (int e = 0; e < R::ACC; ++e) f[e] = r[e];
   }
+#pragma unroll
+  for (int e = 0; e < R::ACC; ++e) acc[(size_t)e * a.n] = f[e];
 }
 
 template <class R, int U>
@@ -1167,10 +1173,10 @@ const SymShape *find_kahan_shape(int unroll, bool tiles) {
   return nullptr;
 }
 
-// Banded launches: super-tile shapes with 4 or 8 particles per lane and at
-// least four super-tiles per item, past their single-launch limit.
+// Banded launches: super-tile shapes with at least four super-tiles per
+// item, past their single-launch limit.
 bool sym_banded(int unroll, bool tiles, int chunk, int acc, size_t n) {
-  return !tiles && (unroll == 4 || unroll == 8) && chunk >= 4 &&
+  return !tiles && (unroll == 2 || unroll == 4 || unroll == 8) && chunk >= 4 &&
          n > sym_max_particles(unroll, tiles, chunk, acc) &&
          n <= kSymBandedMaxParticles;
 }
@@ -1180,13 +1186,25 @@ struct BandShape {
   KernelFn pairs, fold, finish;
 };
 
-// Band kernels of rule R (plain force and adjoint rules; none for Kahan).
+template <class R, int U>
+BandShape band_shape() {
+  return {U, sym_band_kernel<R, U>, sym_band_fold<R, U>, sym_band_finish<R, U>};
+}
+
+// Band kernels of rule R: 2, 4 or 8 particles per lane (Kahan: 2 or 4).
 template <class R>
 const BandShape *find_band_shape(int unroll) {
-  static const BandShape shapes[] = {
-      {4, sym_band_kernel<R, 4>, sym_band_fold<R, 4>, sym_band_finish<R, 4>},
-      {8, sym_band_kernel<R, 8>, sym_band_fold<R, 8>, sym_band_finish<R, 8>},
-  };
+  static const BandShape shapes[] = {band_shape<R, 2>(), band_shape<R, 4>(),
+                                     band_shape<R, 8>()};
+  for (const BandShape &s : shapes)
+    if (s.unroll == unroll) return &s;
+  return nullptr;
+}
+
+template <>
+const BandShape *find_band_shape<KahanForceRule>(int unroll) {
+  static const BandShape shapes[] = {band_shape<KahanForceRule, 2>(),
+                                     band_shape<KahanForceRule, 4>()};
   for (const BandShape &s : shapes)
     if (s.unroll == unroll) return &s;
   return nullptr;
@@ -1265,12 +1283,14 @@ int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
   if ((size_t)n > cap)
     return set_error(
         NB_ERR_UNSUPPORTED,
-        tiles         ? "symmetric tile-pair shapes take at most 2^17 particles"
-        : unroll <= 2 ? "symmetric shapes with 2 per lane take at most 2^18 particles"
+        tiles ? "symmetric tile-pair shapes take at most 2^17 particles"
         : chunk >= 4 && band
             ? "symmetric launch shapes take at most 2^24 particles"
+        : unroll <= 2
+            ? "symmetric shapes with 2 per lane take at most 2^18 particles "
+              "(2^24 with 4 super-tiles per item)"
             : "symmetric launch shapes take at most 2^20 particles (2^24 with "
-              "4 or 8 per lane and 4 super-tiles per item, plain sums)");
+              "4 super-tiles per item; Kahan sums: 2 or 4 per lane)");
   if (!sh || chunk < 1)
     return set_error(NB_ERR_UNSUPPORTED, "no symmetric kernel for this shape");
   if (!scratch)
@@ -1437,8 +1457,10 @@ int launch_forward_sym(const StepConsts &c, int unroll, int chunk, bool tiles,
   a.nz = nz;
   a.c = c;
   if (kahan)
-    return launch_sym(find_kahan_shape(unroll, tiles), a, unroll, chunk, n,
-                      scratch, KahanForceRule::ACC, tiles, stream);
+    return launch_sym(
+        find_kahan_shape(unroll, tiles), a, unroll, chunk, n, scratch,
+        KahanForceRule::ACC, tiles, stream,
+        tiles ? nullptr : find_band_shape<KahanForceRule>(unroll));
   return launch_sym(find_shape<ForceRule>(unroll, tiles), a, unroll, chunk, n,
                     scratch, ForceRule::ACC, tiles, stream,
                     tiles ? nullptr : find_band_shape<ForceRule>(unroll));

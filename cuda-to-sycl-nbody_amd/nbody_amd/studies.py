@@ -103,7 +103,8 @@ def precision(n: int = 1048576, steps: int = 100, seed: int = 7, sample: int = 4
                                                     "split": 8, "scheme": "targets"})
     modes = (("fp32", nb.NB_ACC_PLAIN, -1),            # library default (symmetric)
              ("fp32_targets", nb.NB_ACC_PLAIN, targets),
-             ("fp32_kahan", nb.NB_ACC_KAHAN, -1))
+             ("fp32_kahan", nb.NB_ACC_KAHAN, -1),      # library default (symmetric)
+             ("fp32_kahan_targets", nb.NB_ACC_KAHAN, targets))
     for name, acc, cfg in modes:
         print(f"precision: {name} ...", file=sys.stderr, flush=True)
         V = torch.zeros_like(P0)

@@ -201,12 +201,13 @@ int nb_forward_config_info(int cfg, int *block, int *unroll, int *tile,
  *   NB_SCHEME_TARGETS    each thread sums all sources for its targets;
  *   NB_SCHEME_SYMMETRIC  each pair term is evaluated once and applied to both
  *                        particles (Newton's third law); BRANCH (plain or
- *                        Kahan sums), n <= 2^20 (2^24 with plain sums, 4 or
- *                        8 particles per lane and 4 super-tiles per item, in
- *                        bands of source super-tiles past 2^20, or past
- *                        2^21 with 8 per lane; 2^18 with 2 per lane),
+ *                        Kahan sums), n <= 2^20 (2^18 with 2 per lane; 2^24
+ *                        with 4 super-tiles per item, in bands of source
+ *                        super-tiles past 2^20, past 2^18 with 2 per lane,
+ *                        past 2^21 with 8 per lane and plain sums),
  *                        scratch growing as n^2 (0.4 GB at n = 2^18 with 8
- *                        per lane, 16 GB at 2^21, at most 8 GB per band);
+ *                        per lane, 16 GB at 2^21, at most 8 GB per band,
+ *                        twice that with Kahan sums);
  *                        work item = a pair of 4-tile super-tiles;
  *   NB_SCHEME_SYMMETRIC_TILES  the same with one tile pair per wave, for
  *                        small n (n <= 2^17).

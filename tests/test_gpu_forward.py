@@ -276,7 +276,8 @@ def test_full_size_sampled_steps(nb, orc, gpu, cfg):
 def test_precision_size_sampled_step(nb, orc, gpu, acc):
     """N = 2^20, the precision study's mixture (half Plummer, half disk,
     seed 7): one step of the library's shape (plain: symmetric, 8 per lane,
-    two super-tiles per item; Kahan: target-sharded) on three 2048-target
+    two super-tiles per item; Kahan: symmetric, 2 per lane, four super-tiles
+    per item, in two bands of source super-tiles) on three 2048-target
     ranges against the oracle, and the momentum change of the step against
     the damping-only prediction (fp64 GPU reduction): the symmetric sums
     cancel to rounding."""
@@ -291,7 +292,8 @@ def test_precision_size_sampled_step(nb, orc, gpu, acc):
     if acc == "kahan":
         sim.set_accumulation(nb.NB_ACC_KAHAN)
     info = nb.forward_config_info(sim.config)
-    assert info["scheme"] == ("symmetric" if acc == "plain" else "targets"), info
+    # Kahan: 2 per lane, four super-tiles per item, in two bands
+    assert info["scheme"] == "symmetric", info
     sim.step_sim()
     gp, gv = sim.positions(), sim.velocities()
     sim.close()
