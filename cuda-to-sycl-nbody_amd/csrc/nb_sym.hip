@@ -663,7 +663,8 @@ __device__ __forceinline__ void sum_partials(float (&f)[R::ACC], int count,
 // cut into RUNS contiguous runs), and the first group adds the runs' sums in
 // run order.  The order is fixed either way, so results are deterministic.
 // RUNS = 4 pays where one thread per particle leaves most CUs idle (the
-// tile-pair form: 5.3 vs 10.7 us at n = 12800); with enough particles to
+// tile-pair form: 5.3 vs 10.7 us at n = 12800; 8 and 16 runs made the whole
+// step 0-1 % and 3-7 % slower at n = 4096 .. 40960); with enough particles to
 // fill the chip one run per particle is faster (super-tile finalize at
 // n = 262144: 82 vs 91 us).
 template <class R, int RUNS, class RowAt, class ReacAt>
