@@ -298,7 +298,7 @@ const FwdConfig kAdjConfigs[] = {
     {256, 4, 256, 1, SCHEME_SYMMETRIC_TILES},
     {256, 2, 128, 1, SCHEME_SYMMETRIC_TILES},
     {256, 2, 512, 1, SCHEME_SYMMETRIC}, {256, 2, 512, 2, SCHEME_SYMMETRIC},
-    {256, 4, 1024, 4, SCHEME_SYMMETRIC},
+    {256, 4, 1024, 4, SCHEME_SYMMETRIC}, {256, 2, 512, 4, SCHEME_SYMMETRIC},
 };
 constexpr int kNumAdj = sizeof(kAdjConfigs) / sizeof(kAdjConfigs[0]);
 
@@ -322,11 +322,12 @@ int adjoint_default_config(size_t n, bool skip) {
   // 403.2 / 542.8, 65536 1219 / 1644 (tile pairs 1447 / 1771), 131072
   // 4780 / 6511 vs 4779 / 6478, 262144 19280 / 25892 vs 19168 / 25601
   // (4 per lane 23660 / 31135, 8 per lane 24283 / 34040).  Best
-  // target-sharded: 8192 65.8 / 74.2, 262144 38.7 / 43.5 ms.
-  if (n <= kSymBandedMaxParticles && n > kSymMaxParticles)
-    return 21; // super-tile pairs, 4 per lane, 4 per item, in bands
+  // target-sharded: 8192 65.8 / 74.2, 262144 38.7 / 43.5 ms.  Past 2^18 two
+  // per lane need four super-tiles per item and bands of source super-tiles
+  // (4 per lane at 2^21: 1501 / 1986 ms).
+  if (n <= kSymBandedMaxParticles && n > kSymSmallMaxParticles)
+    return 22; // super-tile pairs, 2 per lane, 4 per item, in bands
   if (n <= kSymMaxParticles) { // skip or not: self pairs masked in-kernel
-    if (n > kSymSmallMaxParticles) return 14; // super-tile pairs, 4 per lane
     if (n >= 98304) return 20; // super-tile pairs, 2 per lane, 2 per item
     if (n >= 20480) return 19; // super-tile pairs, 2 per lane, 1 per item
     if (n >= 1024) return 18;  // tile pairs, 2 per lane
@@ -338,7 +339,7 @@ int adjoint_default_config(size_t n, bool skip) {
 
 size_t adjoint_scratch_bytes(int cfg, size_t n) {
   const FwdConfig &c = kAdjConfigs[cfg];
-  if (c.scheme != SCHEME_TARGETS) { // either variant (band widths differ)
+  if (c.scheme != SCHEME_TARGETS) { // either variant (3 or 6 sums)
     const bool tiles = c.scheme == SCHEME_SYMMETRIC_TILES;
     const size_t fused = sym_scratch_bytes(c.unroll, c.split, n, 6, tiles, true);
     const size_t plain = sym_scratch_bytes(c.unroll, c.split, n, 3, tiles, true);

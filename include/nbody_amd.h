@@ -253,10 +253,10 @@ int nb_adjoint_config_count(void);
 int nb_adjoint_config_info(int cfg, int *block, int *unroll, int *tile,
                            int *split);
 /* Scheme of an adjoint launch shape (NB_SCHEME_*; the symmetric shapes take
- * n <= 2^20, 2^24 with 4 or 8 particles per lane and 4 super-tiles per item
- * (in bands of source super-tiles past 2^20, 16 GB of partials per band for
- * the fused step), 2^18 with 2 particles per lane, 2^17 for tile pairs), or
- * -1 for a bad index. */
+ * n <= 2^20 (2^18 with 2 particles per lane), 2^24 with 4 super-tiles per
+ * item (in bands of source super-tiles past those limits, 16 GB of partials
+ * per band for the fused step), 2^17 for tile pairs), or -1 for a bad
+ * index. */
 int nb_adjoint_config_scheme(int cfg);
 
 /* ---- differentiable trajectory with checkpoints (extension) ------------- */
