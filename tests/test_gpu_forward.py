@@ -627,6 +627,34 @@ def test_symmetric_scheme_rejects_other_modes(nb, gpu):
                         [t[:m] for t in big[3:6]], [t[:m] for t in big[6:]], cfg=small)
 
 
+def test_frame_graph_replays_banded_steps(nb, gpu):
+    """N = 800001 with Kahan sums: the library's shape (2 per lane, four
+    super-tiles per item) runs each step as two bands of source super-tiles
+    (band kernel and fold per band, then the finish kernel).  Frames of 2
+    and 3 steps replayed from captured hipGraphs equal step-by-step launches
+    bit for bit."""
+    n = 800001
+    pos, vel = nb.disk_ic(n)
+    sims = []
+    for graph in (True, False):
+        s = nb.Simulator(nb.default_params(num_particles=n, sim_iterations_per_frame=2),
+                         pos, vel)
+        s.set_accumulation(nb.NB_ACC_KAHAN)
+        s.set_graph(graph)
+        sims.append(s)
+    info = nb.forward_config_info(sims[0].config)
+    assert (info["scheme"], info["unroll"], info["split"]) == ("symmetric", 2, 4), info
+    for k, iters in enumerate((2, 3)):
+        for s in sims:
+            s.set_frame_iterations(iters)
+            s.step_sim()
+        g, d = sims
+        for a, b in zip(g.positions() + g.velocities(), d.positions() + d.velocities()):
+            assert np.array_equal(a, b), (k, iters)
+    for s in sims:
+        s.close()
+
+
 @pytest.mark.parametrize("n", [3000, 25600])
 def test_frame_graph_replays_match_direct_launches(nb, gpu, n):
     """Frames replayed from captured hipGraphs equal step-by-step launches bit
