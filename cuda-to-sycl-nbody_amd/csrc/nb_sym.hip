@@ -777,7 +777,8 @@ __global__ __launch_bounds__(256) void sym_finalize(SymArgs a) {
 // band's row partials (one per item) and reaction partials (one per pair
 // with GB > GA), a fold kernel adds, per particle, its band rows minus its
 // band reactions to a running sum (band order), and after the last band a
-// finish kernel integrates.  The partials of one band stay within a fixed
+// finish kernel applies the rule's epilogue (the integrator; the adjoint's
+// cotangent update).  The partials of one band stay within a fixed
 // budget whatever n is; the order of every sum is fixed.
 
 // Reaction slot of pair (ga, gb), gb > ga, gb in the band [b0, b0 + w).

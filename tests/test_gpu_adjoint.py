@@ -447,6 +447,57 @@ def test_checkpoint_interval_does_not_change_gradient(nb, orc, gpu):
         assert torch.equal(gx, grads[0][0]) and torch.equal(gv, grads[0][1])
 
 
+def test_trajectory_gradient_banded_size(nb, gpu):
+    """N = 300001, T = 3, checkpoints every 2: the trajectory's reverse sweep
+    runs the library's adjoint shape (2 per lane, four super-tiles per item,
+    banded).  Its gradient of a random linear loss of (x_T, v_T) equals a
+    manual reverse sweep through the same forward states with the
+    target-sharded adjoint kernel, to fp32 rounding of the pair sums."""
+    import torch
+
+    n, T = 300001, 3
+    pos, vel = nb.disk_ic(n)
+    x0 = to_dev(torch, pos, gpu)
+    v0 = to_dev(torch, vel, gpu)
+    params = nb.default_params(num_particles=n)
+    rng = np.random.default_rng(13)
+    wx = torch.from_numpy(rng.normal(size=(3, n)).astype(np.float32)).to(gpu)
+    wv = torch.from_numpy(rng.normal(size=(3, n)).astype(np.float32)).to(gpu)
+    x = x0.clone().requires_grad_(True)
+    v = v0.clone().requires_grad_(True)
+    xT, vT = nb.trajectory(x, v, params, T, 2)
+    ((xT * wx).sum() + (vT * wv).sum()).backward()
+    torch.cuda.synchronize()
+    targets = next(c for c in range(nb.adjoint_config_count())
+                   if nb.adjoint_config_info(c)["scheme"] == "targets"
+                   and nb.adjoint_config_info(c)["split"] > 1)
+    P, V, states = x0.clone(), v0.clone(), [x0.clone()]
+    for _ in range(T - 1):
+        N = torch.empty_like(P)
+        nb.forward_step(params, list(P), list(V), list(N))
+        P = N
+        states.append(P)
+
+    def sweep(p):
+        lx, lv = wx.clone(), wv.clone()
+        for t in reversed(range(T)):
+            lx2, lv2 = torch.empty_like(lx), torch.empty_like(lv)
+            nb.adjoint_step(p, states[t], lx, lv, lx2, lv2, cfg=targets)
+            lx, lv = lx2, lv2
+        torch.cuda.synchronize()
+        return lx.double(), lv.double()
+
+    lx, lv = sweep(params)
+    # the same sweep without the pair sums (G = 0): the errors are measured
+    # against the part of the gradient the pair sums make
+    lx0, lv0 = sweep(nb.default_params(num_particles=n, G=0.0))
+    ex = float((x.grad.double() - lx).norm() / (lx - lx0).norm())
+    ev = float((v.grad.double() - lv).norm() / (lv - lv0).norm())
+    print(f"N={n} T={T}: banded vs target-sharded reverse sweep, relative to the "
+          f"pair-sum part: x {ex:.2e} v {ev:.2e}")
+    assert ex < 1e-4 and ev < 1e-4
+
+
 def test_trajectory_central_differences(nb, orc, gpu):
     """Adjoint gradient vs central differences (h = 1e-3) of the fp64 loss
     on 32 random coordinates of x0 and v0 (far target, eps = 1e-2)."""
