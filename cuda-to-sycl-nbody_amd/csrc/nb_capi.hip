@@ -10,6 +10,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -45,41 +46,107 @@ int pair_mode(int calc_method, int acc, float eps) {
 }
 
 namespace {
-struct DeviceScratch {
+// One scratch buffer per (device, stream) pair, grown on demand.  Launches on
+// one stream run in order, so successive calls on a stream share its buffer
+// safely; a call on another stream gets another buffer.  The entry's mutex
+// is held from lookup until the caller has enqueued its launches (the
+// ScratchLease's lifetime), so a second host thread on the same stream waits
+// instead of growing the buffer under the first one's launch.  Growing first
+// waits for the stream (every earlier use of the buffer was on it), then
+// frees.  Entries are never destroyed; nb_scratch_release frees their
+// buffers.
+struct ScratchEntry {
+  int device = 0;
+  hipStream_t stream = nullptr;
   float *ptr = nullptr;
   size_t bytes = 0;
+  std::mutex mu;
 };
-constexpr int kMaxDevices = 64;
-DeviceScratch g_scratch[kMaxDevices];
-std::mutex g_scratch_mu;
+std::mutex g_scratch_mu; // guards g_scratch (the list, not the entries)
+std::vector<std::unique_ptr<ScratchEntry>> g_scratch;
+
+ScratchEntry *scratch_entry(int dev, hipStream_t stream) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  for (auto &e : g_scratch)
+    if (e->device == dev && e->stream == stream) return e.get();
+  g_scratch.push_back(std::make_unique<ScratchEntry>());
+  g_scratch.back()->device = dev;
+  g_scratch.back()->stream = stream;
+  return g_scratch.back().get();
+}
 } // namespace
 
-float *device_scratch(size_t bytes, int *status) {
-  *status = NB_OK;
+int acquire_scratch(size_t bytes, hipStream_t stream, ScratchLease &lease) {
+  lease = ScratchLease();
+  if (bytes == 0) return NB_OK;
   int dev = 0;
-  if ((*status = hip_check(hipGetDevice(&dev), "hipGetDevice"))) return nullptr;
-  if (dev < 0 || dev >= kMaxDevices) {
-    *status = set_error(NB_ERR_UNSUPPORTED, "device index too large");
-    return nullptr;
-  }
-  std::lock_guard<std::mutex> lock(g_scratch_mu);
-  DeviceScratch &s = g_scratch[dev];
-  if (s.bytes < bytes) {
-    if (s.ptr) {
-      // hipFree waits for work that may still read the old buffer.
-      (void)hipFree(s.ptr);
-      s.ptr = nullptr;
-      s.bytes = 0;
+  int st = hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (st) return st;
+  ScratchEntry *e = scratch_entry(dev, stream);
+  std::unique_lock<std::mutex> lock(e->mu);
+  if (e->bytes < bytes) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if ((st = hip_check(hipStreamIsCapturing(stream, &cap), "hipStreamIsCapturing")))
+      return st;
+    if (cap != hipStreamCaptureStatusNone)
+      return set_error(NB_ERR_UNSUPPORTED,
+                       "the call needs " + std::to_string(bytes) +
+                           " bytes of scratch on this stream, more than it holds: "
+                           "make the same call once before capturing the stream");
+    if (e->ptr) {
+      // every earlier use of the buffer was enqueued on this stream
+      if ((st = hip_check(hipStreamSynchronize(stream), "stream synchronize")))
+        return st;
+      (void)hipFree(e->ptr);
+      e->ptr = nullptr;
+      e->bytes = 0;
     }
     void *p = nullptr;
-    if ((*status = hip_check(hipMalloc(&p, bytes), "scratch hipMalloc")))
-      return nullptr;
-    s.ptr = static_cast<float *>(p);
-    s.bytes = bytes;
+    if ((st = hip_check(hipMalloc(&p, bytes), "scratch hipMalloc"))) return st;
+    e->ptr = static_cast<float *>(p);
+    e->bytes = bytes;
   }
-  return s.ptr;
+  lease.ptr_ = e->ptr;
+  lease.lock_ = std::move(lock);
+  return NB_OK;
 }
 
+} // namespace nb
+
+extern "C" int nb_scratch_release(void) {
+  using namespace nb;
+  int dev = 0;
+  int st = hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (st) return st;
+  if ((st = hip_check(hipDeviceSynchronize(), "device synchronize"))) return st;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  for (auto &e : g_scratch) {
+    if (e->device != dev) continue;
+    std::lock_guard<std::mutex> entry_lock(e->mu);
+    if (e->ptr) (void)hipFree(e->ptr);
+    e->ptr = nullptr;
+    e->bytes = 0;
+  }
+  return NB_OK;
+}
+
+extern "C" int nb_scratch_held(size_t *bytes) {
+  using namespace nb;
+  if (!bytes) return set_error(NB_ERR_INVALID_ARG, "bytes is NULL");
+  *bytes = 0;
+  int dev = 0;
+  int st = hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (st) return st;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  for (auto &e : g_scratch) {
+    if (e->device != dev) continue;
+    std::lock_guard<std::mutex> entry_lock(e->mu);
+    *bytes += e->bytes;
+  }
+  return NB_OK;
+}
+
+namespace nb {
 } // namespace nb
 
 using namespace nb;
@@ -94,6 +161,7 @@ struct nb_sim {
   float *pos[3] = {}, *nxt[3] = {}, *vel[3] = {};
   float *scratch = nullptr;
   size_t scratch_bytes = 0;
+  float *vec4 = nullptr; // nb_sim_read_vec4's packing buffer (4 n floats)
   int acc = NB_ACC_PLAIN;
   int cfg = 0;      // launch shape in use
   int cfg_req = -1; // shape asked for with nb_sim_set_config (-1: default)
@@ -149,6 +217,7 @@ void destroy_sim(nb_sim *s) {
   drop_sim_graphs(s);
   if (s->mem) (void)hipFree(s->mem);
   if (s->scratch) (void)hipFree(s->scratch);
+  if (s->vec4) (void)hipFree(s->vec4);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->ev_stage) (void)hipEventDestroy(s->ev_stage);
@@ -413,7 +482,7 @@ int nb_sim_stage_readback(nb_sim *s) {
     return NB_OK;
   }
   int st = NB_OK;
-  if (!s->staging) {
+  if (!s->staging_dev) {
     // pinned, coherent and mapped into the device's address space: the
     // staging kernel stores straight into it
     void *p = nullptr, *d = nullptr;
@@ -421,10 +490,12 @@ int nb_sim_stage_readback(nb_sim *s) {
                                       hipHostMallocMapped),
                         "pinned hipHostMalloc")))
       return st;
-    s->staging = static_cast<float *>(p);
     if ((st = hip_check(hipHostGetDevicePointer(&d, p, 0),
-                        "hipHostGetDevicePointer")))
+                        "hipHostGetDevicePointer"))) {
+      (void)hipHostFree(p); // no half-set-up state is kept
       return st;
+    }
+    s->staging = static_cast<float *>(p);
     s->staging_dev = static_cast<float *>(d);
   }
   if (!s->ev_stage &&
@@ -460,8 +531,13 @@ int nb_sim_read_vec4(nb_sim *s, int which, float w, float *out) {
   if (s->n == 0) return NB_OK;
   if (!out) return set_error(NB_ERR_INVALID_ARG, "output is NULL");
   int st = NB_OK;
-  float *buf = device_scratch(sizeof(float) * 4 * s->n, &st);
-  if (st) return st;
+  if (!s->vec4) { // the handle's own packing buffer, freed with the handle
+    void *p = nullptr;
+    if ((st = hip_check(hipMalloc(&p, sizeof(float) * 4 * s->n), "vec4 hipMalloc")))
+      return st;
+    s->vec4 = static_cast<float *>(p);
+  }
+  float *const buf = s->vec4;
   float *const *src = which == 0 ? s->pos : s->vel;
   if ((st = launch_pack_vec4(src[0], src[1], src[2], w, buf, (int)s->n,
                              s->stream)) ||
@@ -544,18 +620,24 @@ int nb_forward_step_dev(const nb_params *p, const float *px, const float *py,
                      "positions and next positions must not alias");
   const int mode = pair_mode(p->calc_method, acc, p->dist_eps);
   if (cfg < 0) cfg = forward_default_config(n, mode);
-  float *scratch = nullptr;
-  const size_t bytes = forward_scratch_bytes(cfg, n, mode);
-  if (bytes) {
-    scratch = device_scratch(bytes, &st);
-    if (st) return st;
-  }
-  return launch_forward(consts_of(p), mode, cfg, px,
-                        py, pz, vx, vy, vz, nx, ny, nz, (int)n, scratch,
-                        static_cast<hipStream_t>(stream));
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  ScratchLease lease; // held until the launches are enqueued
+  if ((st = acquire_scratch(forward_scratch_bytes(cfg, n, mode), s, lease)))
+    return st;
+  return launch_forward(consts_of(p), mode, cfg, px, py, pz, vx, vy, vz, nx,
+                        ny, nz, (int)n, lease.get(), s);
 }
 
 int nb_forward_config_count(void) { return forward_config_count(); }
+
+int nb_forward_default_config(const nb_params *p, size_t n, int acc) {
+  if (check_params(p)) return -1;
+  if (acc != NB_ACC_PLAIN && acc != NB_ACC_KAHAN) {
+    set_error(NB_ERR_INVALID_ARG, "unknown accumulation mode");
+    return -1;
+  }
+  return forward_default_config(n, pair_mode(p->calc_method, acc, p->dist_eps));
+}
 
 int nb_forward_config_info(int cfg, int *block, int *unroll, int *tile,
                            int *split) {

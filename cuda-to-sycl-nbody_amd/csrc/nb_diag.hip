@@ -191,11 +191,11 @@ int nb_forces_f64_dev(const nb_params *p, const nb_vec3 *pos,
   a.chunk = (int)(((n + chunks - 1) / chunks + kTile - 1) / kTile * kTile);
   chunks = (int)((n + a.chunk - 1) / a.chunk);
   a.eps = (double)p->dist_eps;
-  int st = NB_OK;
-  a.part = reinterpret_cast<double *>(
-      device_scratch(sizeof(double) * 3 * m * (size_t)chunks, &st));
-  if (st) return st;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  ScratchLease part; // held until both launches are enqueued
+  int st = acquire_scratch(sizeof(double) * 3 * m * (size_t)chunks, s, part);
+  if (st) return st;
+  a.part = reinterpret_cast<double *>(part.get());
   void *args[] = {&a};
   st = hip_check(hipLaunchKernel(reinterpret_cast<const void *>(f64_force_partial),
                                  dim3(tblocks, chunks), dim3(kBlock), args, 0, s),
@@ -230,10 +230,10 @@ int nb_energy_momentum_dev(const nb_params *p, const nb_vec3 *pos,
   a.vz = vel->z;
   a.n = (int)n;
   a.eps = (double)p->dist_eps;
-  int st = NB_OK;
-  a.part = reinterpret_cast<double *>(
-      device_scratch(sizeof(double) * 5 * (size_t)blocks, &st));
+  ScratchLease part; // held until both launches are enqueued
+  int st = acquire_scratch(sizeof(double) * 5 * (size_t)blocks, s, part);
   if (st) return st;
+  a.part = reinterpret_cast<double *>(part.get());
   void *args[] = {&a};
   st = hip_check(hipLaunchKernel(reinterpret_cast<const void *>(f64_energy_partial),
                                  dim3(blocks), dim3(kBlock), args, 0, s),

@@ -10,6 +10,7 @@
 #include <cfloat>
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -155,9 +156,25 @@ int launch_forward(const StepConsts &c, int mode, int cfg, const float *px,
 // Bytes of scratch launch_forward needs for (cfg, n, pair mode).
 size_t forward_scratch_bytes(int cfg, size_t n, int mode);
 
-// Per-device scratch owned by the library (grown on demand, never shrunk).
-// Calls on different streams of one device share it.
-float *device_scratch(size_t bytes, int *status);
+// Scratch owned by the library: one buffer per (device, stream), grown on
+// demand and freed by nb_scratch_release.  A lease holds the buffer's lock:
+// keep it alive until the launches that use the buffer are enqueued on the
+// stream it was acquired for.
+class ScratchLease {
+ public:
+  ScratchLease() = default;
+  ScratchLease(ScratchLease &&) = default;
+  ScratchLease &operator=(ScratchLease &&) = default;
+  float *get() const { return ptr_; }
+
+ private:
+  friend int acquire_scratch(size_t, hipStream_t, ScratchLease &);
+  std::unique_lock<std::mutex> lock_;
+  float *ptr_ = nullptr;
+};
+// `bytes` (0: none) of scratch for launches on `stream`.  Refuses to grow a
+// buffer while the stream is being captured into a graph.
+int acquire_scratch(size_t bytes, hipStream_t stream, ScratchLease &lease);
 
 // SoA -> AoS vec4 pack on the device (out: 4 n floats, 16 B aligned).
 // Copies six device arrays of n floats to dst + c * stride (c = 0..5), a

@@ -45,8 +45,12 @@ __global__ __launch_bounds__(256) void stage_six(Six src,
 int launch_stage_six(const float *const src[6], float *dst, int n,
                      size_t stride, hipStream_t stream) {
   if (n <= 0) return NB_OK;
+  if (!dst) return set_error(NB_ERR_INVALID_ARG, "staging destination is NULL");
   Six s;
-  for (int c = 0; c < 6; ++c) s.p[c] = src[c];
+  for (int c = 0; c < 6; ++c) {
+    if (!src[c]) return set_error(NB_ERR_INVALID_ARG, "staging source is NULL");
+    s.p[c] = src[c];
+  }
   const int blocks = std::min((n + 255) / 256, 256 * 2);
   hipLaunchKernelGGL(stage_six, dim3(blocks, 6), dim3(256), 0, stream, s, dst,
                      n, (long long)stride);

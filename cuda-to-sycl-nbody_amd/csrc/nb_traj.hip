@@ -59,11 +59,9 @@ int adjoint_common(const nb_params *p, const nb_vec3 *pos, const nb_vec3 *vel,
       (fused && (overlaps(nxt, pos) || overlaps(vel, pos))))
     return set_error(NB_ERR_INVALID_ARG, "outputs must not alias inputs");
   if (cfg < 0) cfg = adjoint_default_config(n, !self_term_vanishes(p->dist_eps));
-  float *scratch = nullptr;
-  if (const size_t bytes = adjoint_scratch_bytes(cfg, n)) {
-    scratch = device_scratch(bytes, &st);
-    if (st) return st;
-  }
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  ScratchLease scratch; // held until the launches are enqueued
+  if ((st = acquire_scratch(adjoint_scratch_bytes(cfg, n), s, scratch))) return st;
   const float *P[3] = {pos->x, pos->y, pos->z};
   const float *LXN[3] = {lxn->x, lxn->y, lxn->z};
   const float *LVN[3] = {lvn->x, lvn->y, lvn->z};
@@ -74,8 +72,7 @@ int adjoint_common(const nb_params *p, const nb_vec3 *pos, const nb_vec3 *vel,
   float *NX[3] = {fused ? nxt->x : nullptr, fused ? nxt->y : nullptr,
                   fused ? nxt->z : nullptr};
   return launch_adjoint(consts(p), !self_term_vanishes(p->dist_eps), fused,
-                        cfg, P, LXN, LVN, LX, LV, V, NX, (int)n, scratch,
-                        static_cast<hipStream_t>(stream));
+                        cfg, P, LXN, LVN, LX, LV, V, NX, (int)n, scratch.get(), s);
 }
 
 // Workspace layout (floats) of the trajectory passes.
@@ -171,6 +168,14 @@ int nb_adjoint_config_scheme(int cfg) {
     return -1;
   }
   return adjoint_config(cfg).scheme;
+}
+
+int nb_adjoint_default_config(const nb_params *p, size_t n) {
+  if (!p) {
+    set_error(NB_ERR_INVALID_ARG, "params is NULL");
+    return -1;
+  }
+  return adjoint_default_config(n, !self_term_vanishes(p->dist_eps));
 }
 
 size_t nb_traj_checkpoint_floats(size_t n, int steps, int every) {

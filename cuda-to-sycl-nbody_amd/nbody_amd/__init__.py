@@ -20,8 +20,8 @@ import numpy as np
 
 from .diag import energy_momentum, forces_f64
 from .batch import Batch, batch_config_count, batch_config_info, plummer_batch_sizes
-from .adjoint import (adjoint_config_count, adjoint_config_info, adjoint_step,
-                      fwd_adjoint_step, trajectory)
+from .adjoint import (adjoint_config_count, adjoint_config_info, adjoint_default_config,
+                      adjoint_step, fwd_adjoint_step, trajectory)
 from ._lib import (DEFAULT_SEED, DRIVER_PATH, LIB_DIR, LIB_PATH, NB_ACC_KAHAN, NB_ACC_PLAIN,
                    NB_CALC_BRANCH, NB_CALC_PREDICATED, NB_ERR_INVALID_ARG, NB_ERR_NO_DEVICE,
                    NB_IC_LENGTH_F32, NB_IC_TRIG_DOUBLE, NB_OK, SIM_LIB_PATH, NbError, NbParams,
@@ -36,7 +36,32 @@ __all__ = [
     "plummer_ic", "adjoint_config_count", "adjoint_config_info", "adjoint_step",
     "fwd_adjoint_step", "trajectory", "NbVec3", "Batch", "batch_config_count",
     "batch_config_info", "plummer_batch_sizes", "energy_momentum", "forces_f64", "pack_vec4",
+    "scratch_release", "scratch_held", "adjoint_default_config", "forward_default_config",
 ]
+
+
+def forward_default_config(n: int, params: NbParams | None = None,
+                           acc: int = NB_ACC_PLAIN) -> int:
+    """The forward shape the library picks for n particles (SimParam
+    defaults unless params is given)."""
+    p = params if params is not None else default_params(num_particles=n)
+    cfg = int(load().nb_forward_default_config(ctypes.byref(p), n, acc))
+    if cfg < 0:
+        check(NB_ERR_INVALID_ARG)
+    return cfg
+
+
+def scratch_release() -> None:
+    """Frees the library's per-stream scratch buffers on the current device
+    (after waiting for all work on it)."""
+    check(load().nb_scratch_release())
+
+
+def scratch_held() -> int:
+    """Bytes of per-stream scratch the library holds on the current device."""
+    b = ctypes.c_size_t()
+    check(load().nb_scratch_held(ctypes.byref(b)))
+    return int(b.value)
 
 
 def _ptr(a: np.ndarray) -> ctypes.c_void_p:

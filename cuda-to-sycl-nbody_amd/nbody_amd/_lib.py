@@ -157,6 +157,10 @@ def load() -> ctypes.CDLL:
     _sig(lib, "nb_forward_config_count", c_int, [])
     _sig(lib, "nb_forward_config_info", c_int, [c_int] + [POINTER(c_int)] * 4)
     _sig(lib, "nb_forward_config_scheme", c_int, [c_int])
+    _sig(lib, "nb_forward_default_config", c_int, [POINTER(NbParams), c_size_t, c_int])
+    _sig(lib, "nb_adjoint_default_config", c_int, [POINTER(NbParams), c_size_t])
+    _sig(lib, "nb_scratch_release", c_int, [])
+    _sig(lib, "nb_scratch_held", c_int, [POINTER(c_size_t)])
     _lib = lib
     return lib
 

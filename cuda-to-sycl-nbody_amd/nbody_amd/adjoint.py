@@ -55,6 +55,18 @@ def fwd_adjoint_step(params: NbParams, pos, vel, pos_next, lam_x_next, lam_v_nex
                                          n, cfg, _stream(stream)))
 
 
+def adjoint_default_config(n: int, params: NbParams | None = None) -> int:
+    """The adjoint shape the library picks for n particles (SimParam
+    defaults unless params is given)."""
+    from ._lib import default_params
+
+    p = params if params is not None else default_params(num_particles=n)
+    cfg = int(load().nb_adjoint_default_config(ctypes.byref(p), n))
+    if cfg < 0:
+        check(NB_ERR_INVALID_ARG)
+    return cfg
+
+
 def adjoint_config_count() -> int:
     return int(load().nb_adjoint_config_count())
 

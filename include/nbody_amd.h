@@ -194,6 +194,10 @@ int nb_forward_step_dev(const nb_params *p, const float *px, const float *py,
 int nb_forward_config_count(void);
 int nb_forward_config_info(int cfg, int *block, int *unroll, int *tile,
                            int *split);
+/* The shape the library picks (cfg = -1) for n particles with these
+ * parameters and accumulation mode; -1 (and nb_last_error) for bad
+ * arguments. */
+int nb_forward_default_config(const nb_params *p, size_t n, int acc);
 
 /* Decomposition of a forward launch shape (same step, same per-pair
  * arithmetic; no counterpart in the reference, whose kernel is the
@@ -216,6 +220,23 @@ int nb_forward_config_info(int cfg, int *block, int *unroll, int *tile,
 #define NB_SCHEME_SYMMETRIC 1
 #define NB_SCHEME_SYMMETRIC_TILES 2
 int nb_forward_config_scheme(int cfg);
+
+/* Scratch of the nb_*_dev entry points (no reference counterpart: the
+ * reference kernel keeps no partial sums).  Shapes with partial sums
+ * (source-split target-sharded shapes, the symmetric shapes, the fp64
+ * diagnostics) use a buffer the library allocates on first use, one per
+ * (device, stream): calls on one stream reuse it, calls on different streams
+ * never share it, and a call that needs more grows it (after waiting for the
+ * stream).  A call that would grow it while its stream is being captured
+ * into a graph fails with NB_ERR_UNSUPPORTED; make the call once before
+ * capturing.
+ *   nb_scratch_release  waits for all work on the current device, then frees
+ *                       every such buffer of the device (they are allocated
+ *                       again on demand);
+ *   nb_scratch_held     bytes the library holds as such buffers on the
+ *                       current device. */
+int nb_scratch_release(void);
+int nb_scratch_held(size_t *bytes);
 
 
 /* ---- adjoint (extension; no reference counterpart) ---------------------- */
@@ -252,6 +273,9 @@ int nb_fwd_adjoint_step_dev(const nb_params *p, const nb_vec3 *pos,
 int nb_adjoint_config_count(void);
 int nb_adjoint_config_info(int cfg, int *block, int *unroll, int *tile,
                            int *split);
+/* The adjoint shape the library picks (cfg = -1) for n particles with these
+ * parameters; -1 (and nb_last_error) for a NULL p. */
+int nb_adjoint_default_config(const nb_params *p, size_t n);
 /* Scheme of an adjoint launch shape (NB_SCHEME_*; the symmetric shapes take
  * n <= 2^20 (2^18 with 2 particles per lane), 2^24 with 4 super-tiles per
  * item (in bands of source super-tiles past those limits, 16 GB of partials
