@@ -16,7 +16,8 @@ adjoint: the reference disk, a target made by a forward run from initial
   of the final positions; gradient w.r.t. initial positions and velocities
   from the checkpointed adjoint, compared with central differences
   (h = 1e-3, fp32 GPU forward runs) on 32 coordinates.  (The comparison with
-  an fp64 CPU adjoint is tests/adjoint_fp64_check.py.)
+  an fp64 CPU adjoint at this workload is a GPU test,
+  tests/test_gpu_workloads.py.)
 shapes: every compiled launch shape timed at each size (forward step plain
   and Kahan, adjoint plain and fused), fastest first: the measurements the
   library's default shapes come from.

@@ -2,18 +2,23 @@
 restatement of particle_interaction, src/simulator.cu:186-229).
 
 Error metric (per particle): max_c |x_gpu - x_ref| / max(max_c |x_ref|, 1),
-reported as max and 99th percentile over particles.  Tolerance: 1e-5 for one
-step from an identical state (the per-step target); the GPU differs from the
-oracle only by rounding (hardware rsqrt, fused multiply-adds, tiled order).
-Where a particle's force sum is ill-conditioned (a close encounter with
-eps = 1e-7), two correct fp32 evaluations can differ by more than that; for
-such particles fp64 is the referee: the GPU's one-step result must then be no
-farther from the fp64 step than the fp32 oracle's is (within a factor of 4);
-the referee is applied to every particle that differs by more than 1e-6, and
-the 99th percentile must stay within 1e-5.
-Over many steps close encounters amplify rounding the same way they amplify
-fp32-vs-fp64 differences, so multi-step trajectories are compared
-statistically.
+reported as max and 99th percentile over particles.
+
+One step from an identical state: the per-step target is 1e-5 against the
+oracle; the GPU differs from it only by rounding (hardware rsqrt, fused
+multiply-adds, tiled summation order).  Where a particle's force sum is
+ill-conditioned (a close encounter with eps = 1e-7) two correct fp32
+evaluations can differ by more than that, so every particle where the GPU
+and the oracle differ by more than 1e-6 is refereed by the fp64 step from
+the same fp32 state: the GPU must be no farther from it than the oracle is
+(plus 1e-7 absolute); the 99th percentile of the GPU's own distance from
+the fp64 step must stay within 1e-5.
+
+Ten steps from the initial conditions: close encounters amplify rounding
+chaotically (the oracle itself ends up to 7e-2 from fp64 at N = 25600), so
+the trajectories are refereed by an fp64 trajectory of all particles from
+the same initial state: the GPU's max and 99th percentile distance from it
+must be no larger than the fp32 oracle's.
 """
 import numpy as np
 import pytest
@@ -21,13 +26,6 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 TOL_STEP = 1e-5
-
-
-def rel_err(got, ref):
-    g = np.stack(got).astype(np.float64)
-    r = np.stack(ref).astype(np.float64)
-    e = np.abs(g - r).max(0) / np.maximum(np.abs(r).max(0), 1.0)
-    return float(e.max()), float(np.percentile(e, 99))
 
 
 def per_particle(got, ref):
@@ -50,10 +48,10 @@ def assert_step_parity(orc, p, pos, vel, gpu_pos, ref_pos, idx, label):
     """gpu_pos/ref_pos: positions after one step at targets idx.
 
     Every particle where the GPU and the fp32 oracle differ by more than 1e-6
-    is refereed by the fp64 step: the GPU may be no farther from it than 4x
-    the oracle's distance.  The 99th percentile of the GPU's own error
-    against the fp64 step must stay within the per-step target (where the
-    two fp32 results agree to 1e-6, the oracle's distance is used)."""
+    is refereed by the fp64 step: the GPU may be no farther from it than the
+    oracle is (+1e-7).  The 99th percentile of the GPU's own error against
+    the fp64 step must stay within the per-step target (where the two fp32
+    results agree to 1e-6, the oracle's distance is used)."""
     e_go = per_particle(gpu_pos, ref_pos)
     bad = np.nonzero(e_go > 1e-6)[0]
     msg = f"{label}: vs fp32 oracle max {e_go.max():.2e} p99 {np.percentile(e_go, 99):.2e}"
@@ -62,9 +60,11 @@ def assert_step_parity(orc, p, pos, vel, gpu_pos, ref_pos, idx, label):
         x64 = fp64_step(orc, p, pos, vel, idx[bad])
         e_g64 = per_particle([a[bad] for a in gpu_pos], x64)
         e_o64 = per_particle([a[bad] for a in ref_pos], x64)
-        msg += (f"; {bad.size} over 1e-6: vs fp64 gpu {e_g64.max():.2e} "
-                f"oracle {e_o64.max():.2e}")
-        assert np.all(e_g64 <= 4 * e_o64 + 1e-7), msg
+        worse = int(np.count_nonzero(e_g64 > e_o64 + 1e-7))
+        msg += (f"; {bad.size} over 1e-6: vs fp64 gpu max {e_g64.max():.2e} "
+                f"p99 {np.percentile(e_g64, 99):.2e}, oracle max {e_o64.max():.2e} "
+                f"p99 {np.percentile(e_o64, 99):.2e}; gpu farther at {worse}")
+        assert worse == 0, msg
         e_gpu[bad] = e_g64
     msg += f"; gpu p99 vs fp64 <= {np.percentile(e_gpu, 99):.2e}"
     print(msg)
@@ -93,6 +93,21 @@ def test_one_step_config1(nb, orc, gpu):
     assert_step_parity(orc, p, pos, vel, gp, rp, np.arange(n), f"N={n} step 1")
 
 
+def assert_trajectory_parity(gpu_pos, ref_pos, pos64, label):
+    """Positions after the same number of steps from the same initial state:
+    the GPU's max and 99th-percentile distance from the fp64 trajectory are
+    no larger than the fp32 oracle's."""
+    e_g = per_particle(gpu_pos, pos64)
+    e_o = per_particle(ref_pos, pos64)
+    e_go = per_particle(gpu_pos, ref_pos)
+    msg = (f"{label}: vs fp64 gpu max {e_g.max():.2e} p99 {np.percentile(e_g, 99):.2e}, "
+           f"oracle max {e_o.max():.2e} p99 {np.percentile(e_o, 99):.2e}; gpu vs oracle max "
+           f"{e_go.max():.2e} p99 {np.percentile(e_go, 99):.2e}")
+    print(msg)
+    assert e_g.max() <= e_o.max(), msg
+    assert np.percentile(e_g, 99) <= np.percentile(e_o, 99), msg
+
+
 def test_tenth_step_and_trajectory_config1(nb, orc, gpu):
     n = 25600
     pos, vel = nb.disk_ic(n)
@@ -104,11 +119,12 @@ def test_tenth_step_and_trajectory_config1(nb, orc, gpu):
     gp, _ = gpu_steps(nb, nb.default_params(num_particles=n), rp, rv, 1)
     rp10, _ = orc.step_f32(p, rp, rv)
     assert_step_parity(orc, p, rp, rv, gp, rp10, np.arange(n), f"N={n} step 10")
-    # ten steps from the initial conditions: chaotic divergence, bounded
+    # ten steps from the initial conditions, refereed by the fp64 trajectory
     gp10, _ = gpu_steps(nb, nb.default_params(num_particles=n), pos, vel, 10)
-    tmax, t99 = rel_err(gp10, rp10)
-    print(f"N={n} 10-step trajectory: max rel {tmax:.2e} p99 {t99:.2e}")
-    assert t99 <= 1e-4 and tmax < 0.5
+    x64, v64 = pos, vel
+    for _ in range(10):
+        x64, v64 = orc.step_f64(p, x64, v64)
+    assert_trajectory_parity(gp10, rp10, x64, f"N={n} 10-step trajectory")
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 63, 64, 65, 255, 257, 1023, 1025, 4097])
@@ -222,7 +238,8 @@ def test_frame_timing(nb, gpu):
     sim.step_sim()
     assert 0 < sim.last_kernel_ms <= sim.last_step_ms * 1.05 + 0.05
     sim.close()
-    assert "MI3" in nb.device_name() or nb.device_name() != "Unknown Device"
+    # the library is built for gfx950 only: the name carries the part's ISA
+    assert "gfx950" in nb.device_name(), nb.device_name()
 
 
 def sym_configs(nb):
@@ -270,6 +287,74 @@ def test_full_size_sampled_steps(nb, orc, gpu, cfg):
         if step == 9:
             prev = (sim.positions(), sim.velocities())
     sim.close()
+
+
+def forces64_torch(torch, X, eps, chunk=512):
+    """fp64 all-pairs force sum_j (x_j - x_i) (|r|^2 + eps)^-3/2 in plain
+    PyTorch (float64 throughout, X: (3, n) float64 CUDA tensor), chunks of
+    `chunk` targets.  The self term is 0 for eps > 0."""
+    n = X.shape[1]
+    F = torch.empty_like(X)
+    for i0 in range(0, n, chunk):
+        xi = X[:, i0:i0 + chunk]
+        r = [X[c][None, :] - xi[c][:, None] for c in range(3)]
+        d = r[0] * r[0]
+        d.addcmul_(r[1], r[1]).addcmul_(r[2], r[2]).add_(eps)
+        K = torch.rsqrt(d * d * d)
+        for c in range(3):
+            F[c, i0:i0 + chunk] = (r[c] * K).sum(1)
+    return F
+
+
+def fp64_trajectory_torch(torch, gpu, p, pos, vel, steps):
+    """The integrator of src/simulator.cu:214-228 in fp64 (the parameters'
+    float32 values) over `steps` steps of forces64_torch."""
+    X = torch.from_numpy(np.stack(pos).astype(np.float64)).to(gpu)
+    V = torch.from_numpy(np.stack(vel).astype(np.float64)).to(gpu)
+    f = lambda v: float(np.float32(v))
+    eps, damping, dt, G = f(p.dist_eps), f(p.damping), f(p.dt), f(p.G)
+    for _ in range(steps):
+        F = forces64_torch(torch, X, eps)
+        V = damping * V + (dt * G) * F
+        X = X + dt * V
+    return [a for a in X.cpu().numpy()]
+
+
+def test_full_size_trajectory(nb, orc, gpu):
+    """N = 262144: ten steps from the initial conditions with the library's
+    default shape (symmetric) and the target-sharded shape, against the fp32
+    oracle's ten steps, both refereed by an fp64 trajectory of all particles
+    (plain PyTorch float64 on the GPU, its force checked here against the
+    oracle's fp64 force)."""
+    import torch
+
+    n = 262144
+    pos, vel = nb.disk_ic(n)
+    p = orc.params(num_particles=n)
+    # the fp64 referee's force against the oracle's fp64 force on a sample
+    idx = np.arange(0, n, 509)
+    F = forces64_torch(torch, torch.from_numpy(np.stack(pos).astype(np.float64)).to(gpu),
+                       float(np.float32(p.dist_eps)))
+    Fs = F[:, torch.from_numpy(idx).to(gpu)].cpu().numpy()
+    Fo = np.stack(orc.forces_f64(pos, idx))
+    ef = np.abs(Fs - Fo).max(0) / np.abs(Fo).max(0)
+    print(f"fp64 referee force vs oracle fp64 force on {idx.size} targets: max {ef.max():.2e}")
+    assert ef.max() <= 1e-10
+    del F
+    x64 = fp64_trajectory_torch(torch, gpu, p, pos, vel, 10)
+    rp, rv = pos, vel
+    for _ in range(10):
+        rp, rv = orc.step_f32(p, rp, rv)
+    for cfg in (-1, target_config(nb, n)):
+        params = nb.default_params(num_particles=n, sim_iterations_per_frame=10)
+        sim = nb.Simulator(params, pos, vel)
+        if cfg >= 0:
+            sim.set_config(cfg)
+        label = nb.forward_config_info(sim.config)["scheme"]
+        sim.step_sim()
+        gp = sim.positions()
+        sim.close()
+        assert_trajectory_parity(gp, rp, x64, f"N={n} 10-step trajectory ({label})")
 
 
 @pytest.mark.parametrize("acc", ["plain", "kahan"])
