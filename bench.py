@@ -91,6 +91,19 @@ def _events(torch, fn, reps: int) -> float:
     return e0.elapsed_time(e1) / reps
 
 
+# FLOP credited per interaction to the fused forward + adjoint step: the
+# forward step's 20 plus the adjoint's own arithmetic per interaction in the
+# reference's target-by-target form (h = g_j - g_k: 3; r . h: 5; 1/d: 2;
+# t3 = -3 (r . h) / d: 2; u = t3 r + h: 6; lambda += K u: 6) = 24.
+FUSED_FLOP_PER_INTERACTION = 20 + 24
+# FLOP the hardware executes per interaction in the fused step (FMA = 2,
+# v_rsq_f32 = 1): target-sharded 42 (the forward's 18 + 24); symmetric, per
+# pair term (two interactions): r 3, d 6, d^3 2, rsq 1, 1/d 2, h 3, r . h 5,
+# t3 2, u 6, target and reaction sums of K u 12, of r K 12 = 54, i.e. 27.
+FUSED_EXECUTED_FLOP_PER_INTERACTION = {"targets": 42, "symmetric": 27,
+                                       "symmetric_tiles": 27}
+
+
 def fwd_adjoint_section(nb, torch, pos, vel, n: int, fwd_ms: float) -> dict:
     import numpy as np
 
@@ -108,10 +121,24 @@ def fwd_adjoint_section(nb, torch, pos, vel, n: int, fwd_ms: float) -> dict:
     fused_ms = _events(torch, lambda: nb.fwd_adjoint_step(params, P, V, N, lxn, lvn, lx, lv),
                        reps)
     inter = float(n) * n
+    scheme = nb.adjoint_config_info(nb.adjoint_default_config(n))["scheme"]
+    credited = FUSED_FLOP_PER_INTERACTION * inter / (fused_ms * 1e-3) / 1e12
+    executed = FUSED_EXECUTED_FLOP_PER_INTERACTION[scheme] * inter / (fused_ms * 1e-3) / 1e12
     return {
         "value": round(inter / (fused_ms * 1e-3) / 1e9, 2),
         "unit": "Ginteractions/s",
         "ms_per_step": round(fused_ms, 4),
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(credited, 2),
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(credited / PEAK_FP32_TFLOPS, 4),
+            "flop_per_interaction": FUSED_FLOP_PER_INTERACTION,
+            "scheme": scheme,
+            "executed_flop_per_interaction": FUSED_EXECUTED_FLOP_PER_INTERACTION[scheme],
+            "executed_frac": round(executed / PEAK_FP32_TFLOPS, 4),
+        },
         "ratio_to_fwd": round(fused_ms / fwd_ms, 3),
         "target_ratio": 3.5,
         "adjoint_only_ms": round(adj_ms, 4),
@@ -210,7 +237,45 @@ def traffic_from_profile(scheme: str, n: int):
 EXECUTED_FLOP_PER_INTERACTION = {"targets": 18, "symmetric": 12, "symmetric_tiles": 12}
 
 
-def cpu_baseline(pos, vel, n: int, seconds: float) -> dict:
+def host_cpu() -> dict:
+    """The host's CPU model, its logical CPUs and the threads this process may
+    use (the oracle's OpenMP threads follow OMP_NUM_THREADS when set)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f
+                          if ln.startswith("model name")), None)
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_config1(nb, orc, frames: int = 10, warm: int = 2) -> dict:
+    """The CPU configuration (N = 25600 reference disk, SimParam defaults, 10
+    steps) timed the reference driver's way (src/nbody.cpp:83-123): one step
+    per frame, wall time per frame, the first `warm` frames discarded, mean
+    and population standard deviation of the rest."""
+    n = 25600
+    pos, vel = nb.disk_ic(n)
+    p = orc.params(num_particles=n)
+    times = []
+    for _ in range(frames):
+        t0 = time.perf_counter()
+        pos, vel = orc.step_f32(p, pos, vel)
+        times.append((time.perf_counter() - t0) * 1e3)
+    kept = times[warm:]
+    mean = sum(kept) / len(kept)
+    std = (sum((t - mean) ** 2 for t in kept) / len(kept)) ** 0.5
+    return {"n": n, "steps": frames, "steps_per_frame": 1, "warmup_frames": warm,
+            "mean_ms": round(mean, 3), "stddev_ms": round(std, 3),
+            "value": round(float(n) * n / (mean * 1e-3) / 1e9, 3), "unit": "Ginteractions/s",
+            "line": f"At step {frames} kernel time is {kept[-1]:g} and mean is {mean:g} and "
+                    f"stddev is: {std:g}"}
+
+
+def cpu_baseline(nb, pos, vel, n: int, seconds: float) -> dict:
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as orc
 
@@ -231,6 +296,8 @@ def cpu_baseline(pos, vel, n: int, seconds: float) -> dict:
         "kind": "port",
         "sample": f"one fp32 step of targets [0,{targets}) of N={n} against all {n} sources "
                   f"(reference-order restatement, OpenMP over targets, {dt:.1f} s)",
+        "host": host_cpu(),
+        "config1": cpu_config1(nb, orc),
     }
 
 
@@ -300,15 +367,15 @@ def main() -> int:
             "launch": cfg,
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "valu",
             "achieved": round(achieved, 2),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
             "traffic": traffic_from_profile(cfg["scheme"], n),
-            "bound_note": "fp32 compute roof: 157.3 TFLOP/s is MI355X's fp32 peak for the "
-                          "vector ALUs and the matrix cores alike; the pair loop runs on the "
-                          "vector ALUs (not a contraction)",
+            "bound_note": "fp32 vector ALU roof: 157.3 TFLOP/s, MI355X's fp32 vector peak "
+                          "(two wave64 FMA issues per SIMD every 2 clocks); the pair loop runs "
+                          "on the vector ALUs, not the matrix cores (it is not a contraction)",
             "flop_per_interaction": FLOP_PER_INTERACTION,
             "kernel_ms_per_launch": round(per_launch_ms, 4),
             "executed_frac": round(EXECUTED_FLOP_PER_INTERACTION[cfg["scheme"]] * inter
@@ -320,7 +387,7 @@ def main() -> int:
         line["trajectory"] = trajectory_section(nb, torch, pos, vel, n, per_launch_ms)
         line["batched"] = batched_section(nb, args.batch_steps)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(pos, vel, n, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(nb, pos, vel, n, args.cpu_seconds)
     print(json.dumps(line), flush=True)
     return 0
 
