@@ -72,12 +72,13 @@ coords_t length(const vec3 v);
 vec3 cross(const vec3 v0, const vec3 v1);
 vec3 normalize(const vec3 v);
 
-// Interface of src/simulator.cuh:107-116.  The CUDA version's last virtual is
-// getGwSize(), the SYCL twin's is getCM() (src_sycl/simulator.dp.hpp:116-125);
-// both are provided.
+// Interface of src/simulator.cuh:107-116: the seven pure virtuals of the
+// CUDA version, nothing more, so a class written against the reference
+// interface compiles against this one.  getCM() is a non-virtual member of
+// DiskGalaxySimulator, as at src/simulator.cuh:140 (the SYCL twin's
+// interface has it in place of getGwSize, src_sycl/simulator.dp.hpp:116-125).
 class Simulator {
 public:
-  virtual ~Simulator() {}
   virtual void stepSim() = 0;
   virtual size_t getNumParticles() = 0;
   virtual const ParticleData &getParticlePos() = 0;
@@ -85,7 +86,6 @@ public:
   virtual float getLastStepTime() = 0;
   virtual const std::string *getDeviceName() = 0;
   virtual int getGwSize() = 0;
-  virtual CalculationMethod getCM() = 0;
 };
 
 // src/simulator.cuh:129-160 on MI355X.  State stays on the device between
