@@ -1,5 +1,6 @@
 """Builds libnbody_amd (HIP kernels + C ABI), libnbody_sim (C++ Simulator
-over the C ABI) and the nbody_amd driver, in-tree under lib/.
+over the C ABI), the nbody_amd driver and the nb_issue_rates
+microbenchmark, in-tree under lib/.
 
     python cuda-to-sycl-nbody_amd/build.py
 
@@ -74,6 +75,10 @@ def build(verbose: bool = True) -> Path:
         # --save-temps writes the gfx950 assembly next to the object (cwd)
         _run([HIPCC, *HIP_FLAGS, "--save-temps", "-c", CSRC / src, "-o", obj], cwd=obj_dir)
         objs.append(obj)
+    # the issue-rate microbenchmark of the pair loop's instructions (a program)
+    probe = LIB / "nb_issue_rates"
+    _run([HIPCC, *HIP_FLAGS, "--save-temps", PKG / "probe" / "issue_rates.hip", "-o", probe],
+         cwd=obj_dir)
     import isa_check  # scans the gfx950 assembly written by --save-temps
 
     isa_check.check_dir(obj_dir, ARCH)
