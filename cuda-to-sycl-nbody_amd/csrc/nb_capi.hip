@@ -18,6 +18,13 @@
 
 #include "nb_internal.hpp"
 
+// Scratch of nb_forward_step_noalloc_dev: a file-scope device array, present
+// on every device the library runs on for the life of the process.  In host
+// code `scratch` names a host-side stand-in; the device address comes from
+// hipGetSymbolAddress (static_scratch below), once per device.
+constexpr size_t kStaticScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) float scratch[kStaticScratchBytes / sizeof(float)];
+
 namespace nb {
 
 namespace {
@@ -600,17 +607,25 @@ int nb_sim_destroy(nb_sim *s) {
   return NB_OK;
 }
 
-int nb_forward_step_dev(const nb_params *p, const float *px, const float *py,
-                        const float *pz, float *vx, float *vy, float *vz,
-                        float *nx, float *ny, float *nz, size_t n, int cfg,
-                        int acc, void *stream) {
+} // extern "C"
+
+namespace {
+
+// Argument checks of the stream-ordered forward entry points; on success
+// *mode and *cfg hold the pair mode and the launch shape (the library's
+// choice for cfg = -1).  n == 0 passes (nothing to launch).
+int check_forward_dev(const nb_params *p, const float *px, const float *py,
+                      const float *pz, const float *vx, const float *vy,
+                      const float *vz, const float *nx, const float *ny,
+                      const float *nz, size_t n, int acc, int *cfg,
+                      int *mode) {
   int st = check_params(p);
   if (st) return st;
   if (n > kMaxParticles)
     return set_error(NB_ERR_INVALID_ARG, "n exceeds 2^28");
   if (acc != NB_ACC_PLAIN && acc != NB_ACC_KAHAN)
     return set_error(NB_ERR_INVALID_ARG, "unknown accumulation mode");
-  if (cfg < -1 || cfg >= forward_config_count())
+  if (*cfg < -1 || *cfg >= forward_config_count())
     return set_error(NB_ERR_INVALID_ARG, "forward config out of range");
   if (n == 0) return NB_OK;
   if (!px || !py || !pz || !vx || !vy || !vz || !nx || !ny || !nz)
@@ -618,14 +633,85 @@ int nb_forward_step_dev(const nb_params *p, const float *px, const float *py,
   if (px == nx || py == ny || pz == nz)
     return set_error(NB_ERR_INVALID_ARG,
                      "positions and next positions must not alias");
-  const int mode = pair_mode(p->calc_method, acc, p->dist_eps);
-  if (cfg < 0) cfg = forward_default_config(n, mode);
+  *mode = pair_mode(p->calc_method, acc, p->dist_eps);
+  if (*cfg < 0) *cfg = forward_default_config(n, *mode);
+  return NB_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int nb_forward_step_dev(const nb_params *p, const float *px, const float *py,
+                        const float *pz, float *vx, float *vy, float *vz,
+                        float *nx, float *ny, float *nz, size_t n, int cfg,
+                        int acc, void *stream) {
+  int mode = 0;
+  int st = check_forward_dev(p, px, py, pz, vx, vy, vz, nx, ny, nz, n, acc,
+                             &cfg, &mode);
+  if (st || n == 0) return st;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   ScratchLease lease; // held until the launches are enqueued
   if ((st = acquire_scratch(forward_scratch_bytes(cfg, n, mode), s, lease)))
     return st;
   return launch_forward(consts_of(p), mode, cfg, px, py, pz, vx, vy, vz, nx,
                         ny, nz, (int)n, lease.get(), s);
+}
+
+} // extern "C"
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_static_mu;
+float *g_static_ptr[kMaxDevices] = {}; // device address of `scratch`, per device
+
+// Device address of the static scratch array on the current device, looked
+// up on the first call on that device.
+int static_scratch(float **out) {
+  *out = nullptr;
+  int dev = 0;
+  int st = hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (st) return st;
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_error(NB_ERR_UNSUPPORTED, "device index too large");
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (!g_static_ptr[dev]) {
+    void *ptr = nullptr;
+    if ((st = hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)),
+                        "hipGetSymbolAddress(scratch)")))
+      return st;
+    g_static_ptr[dev] = static_cast<float *>(ptr);
+  }
+  *out = g_static_ptr[dev];
+  return NB_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int nb_forward_step_noalloc_dev(const nb_params *p, const float *px,
+                                const float *py, const float *pz, float *vx,
+                                float *vy, float *vz, float *nx, float *ny,
+                                float *nz, size_t n, int cfg, int acc,
+                                void *stream) {
+  int mode = 0;
+  int st = check_forward_dev(p, px, py, pz, vx, vy, vz, nx, ny, nz, n, acc,
+                             &cfg, &mode);
+  if (st || n == 0) return st;
+  const size_t bytes = forward_scratch_bytes(cfg, n, mode);
+  if (bytes > kStaticScratchBytes)
+    return set_error(NB_ERR_UNSUPPORTED,
+                     "the launch shape needs " + std::to_string(bytes) +
+                         " bytes of scratch at n = " + std::to_string(n) +
+                         ", more than the 64 MiB static buffer of the "
+                         "allocation-free step: use nb_forward_step_dev or a "
+                         "shape with fewer partial sums");
+  float *buf = nullptr;
+  if (bytes && (st = static_scratch(&buf))) return st;
+  return launch_forward(consts_of(p), mode, cfg, px, py, pz, vx, vy, vz, nx,
+                        ny, nz, (int)n, buf, static_cast<hipStream_t>(stream));
 }
 
 int nb_forward_config_count(void) { return forward_config_count(); }

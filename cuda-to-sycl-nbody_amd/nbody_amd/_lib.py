@@ -120,6 +120,8 @@ def load() -> ctypes.CDLL:
     _sig(lib, "nb_sim_destroy", c_int, [vp])
     _sig(lib, "nb_forward_step_dev", c_int,
          [POINTER(NbParams)] + [vp] * 9 + [c_size_t, c_int, c_int, vp])
+    _sig(lib, "nb_forward_step_noalloc_dev", c_int,
+         [POINTER(NbParams)] + [vp] * 9 + [c_size_t, c_int, c_int, vp])
     V3 = POINTER(NbVec3)
     _sig(lib, "nb_adjoint_step_dev", c_int,
          [POINTER(NbParams), V3, V3, V3, V3, V3, c_size_t, c_int, vp])

@@ -187,6 +187,25 @@ int nb_forward_step_dev(const nb_params *p, const float *px, const float *py,
                         float *nx, float *ny, float *nz, size_t n, int cfg,
                         int acc, void *stream);
 
+/* nb_forward_step_dev without any allocation (no counterpart in the
+ * reference: it is for callers that cannot allocate device memory, such as
+ * code under graph capture or a tight serving loop).  The same arguments,
+ * shapes and results as nb_forward_step_dev, bit for bit; the partial sums
+ * of shapes that keep them live in a static 64 MiB device buffer of the
+ * library (one per GPU, held until the process ends), which covers the
+ * default shapes of plain sums up to n = 26624 (the reference's default
+ * 12800 and the 25600 of its CPU configuration among them; Kahan sums up to
+ * n = 18816).  Nothing in it needs clearing between calls: every partial
+ * sum is written before it is read.  A shape whose partial sums
+ * exceed it is refused with NB_ERR_UNSUPPORTED.  Every call on one GPU uses
+ * that GPU's buffer: calls that overlap on one GPU (different streams or
+ * host threads) race and must be ordered by the caller. */
+int nb_forward_step_noalloc_dev(const nb_params *p, const float *px,
+                                const float *py, const float *pz, float *vx,
+                                float *vy, float *vz, float *nx, float *ny,
+                                float *nz, size_t n, int cfg, int acc,
+                                void *stream);
+
 /* Number of compiled forward launch shapes, and a description of one:
  * block threads, targets per thread, LDS tile length, j-split count.
  * For a symmetric shape: 256 threads, particles per lane, super-tile length,
