@@ -26,9 +26,9 @@
 //   * for GB > GA, in pass b wave w meets source tile j = (w + b) & 3; in
 //     each of 64 steps lane l takes the U sources of set (l + s) & 63 from
 //     LDS and evaluates the U x U pair terms, adding the (negated) reaction
-//     to U per-source sums that rotate one lane down the wave (DPP wave_rol:1)
-//     after each step, so that after 64 steps lane l again holds set l's
-//     sums.  Between passes the reaction sums of the four tiles live in LDS
+//     to U per-source sums that rotate one lane down the wave (wave_rol1 or
+//     lds_rol1) after each step, so that after 64 steps lane l again holds
+//     set l's sums.  Between passes the reaction sums of the four tiles live in LDS
 //     (each tile is owned by one wave per pass); after pass 3 the chunk
 //     writes them as the (GA, GB) reaction partial;
 //   * for GB == GA (first super-tile of a row) each wave meets its own tile
@@ -55,11 +55,12 @@
 // its own tile, the only one holding self pairs, masks them as the
 // reference's `continue` skips them.
 //
-// Per pair term (two interactions): forward 14 VALU + v_rsq_f32 + 3/U DPP
-// moves; adjoint 25 VALU + v_rsq_f32 (K and 1/d from rsqrt(d)) + 3/U DPP;
+// Per pair term (two interactions): forward 14 VALU + v_rsq_f32 + 3/U
+// reaction moves through the LDS crossbar (ds_bpermute, no VALU slot);
+// adjoint 25 VALU + v_rsq_f32 (K and 1/d from rsqrt(d)) + 3/U DPP moves;
 // fused adjoint 33 VALU + v_rsq_f32 (K as the forward rule forms it, 1/d =
-// K^2 d^2) + 6/U DPP.  The target-sharded kernels spend 11 (forward) and 22
-// (adjoint) VALU per interaction.
+// K^2 d^2) + 6/U DPP moves.  The target-sharded kernels spend 11 (forward)
+// and 22 (adjoint) VALU per interaction.
 //
 // Build with -ffp-contract=off (explicit fmaf only).
 
@@ -71,16 +72,39 @@ namespace nb {
 namespace {
 
 constexpr int kWaves = 4;        // waves per workgroup = tiles per super-tile
+constexpr size_t kTilePrioMaxParticles = 12800; // see launch_forward_sym
 constexpr float kFar = 1.0e30f;  // coordinate of ForceRule padding particles
 constexpr int kDppWaveRol1 = 0x134;
 
-// Lane l receives lane (l + 1) mod 64's value.  Every lane is written (full
-// row and bank masks), so the "old" operand is the source itself: with a
-// constant there the compiler materialises it with an extra v_mov per call.
+// Lane l receives lane (l + 1) mod 64's value.  Two forms of the same move:
+//   wave_rol1  a DPP wave_rol:1 move on the VALU.  Every lane is written
+//              (full row and bank masks), so the "old" operand is the source
+//              itself: with a constant there the compiler materialises it
+//              with an extra v_mov per call;
+//   lds_rol1   ds_bpermute_b32 through the LDS crossbar (no LDS memory): it
+//              takes no VALU issue slot.  Inside the pair loop's mix a DPP
+//              move costs about as much VALU time as a v_rsq_f32 (10-14 SIMD
+//              cycles, measured with probe/issue_rates.hip), while the
+//              permute's latency is hidden (each sum is rotated as soon as
+//              it is complete and read again only in the next step).  The
+//              forward rules rotate with lds_rol1 (N = 262144: 11.3 vs 11.9
+//              ms with plain sums, 22.5 vs 23.8 ms with Kahan sums; the
+//              64-system batch 4.43 vs 4.52 ms); the adjoint rules keep DPP,
+//              whose loops read twice the LDS per step already (lds_rol1
+//              measured 1.5 % slower for the adjoint, 4 % for the fused
+//              step).
 __device__ __forceinline__ float wave_rol1(float v) {
   const int b = __builtin_bit_cast(int, v);
   return __builtin_bit_cast(
       float, __builtin_amdgcn_update_dpp(b, b, kDppWaveRol1, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float lds_rol1(float v) {
+  const int lane =
+      (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_ds_bpermute(((lane + 1) & 63) << 2,
+                                          __builtin_bit_cast(int, v)));
 }
 
 // sum_{l=1..m} ceil(l / L): work items of the rows of lengths 1..m.
@@ -156,6 +180,7 @@ struct ForceRule : PlainPartials<3> {
   static constexpr int SRC = 3; // x, y, z
   static constexpr int ACC = 3; // F
   static constexpr bool MASKED_TAIL = false;
+  static constexpr bool ROTATE_LDS = true; // lds_rol1 (see wave_rol1)
 
   static __device__ __forceinline__ void load(const SymArgs &a, int j,
                                               float (&v)[SRC]) {
@@ -210,6 +235,7 @@ struct KahanForceRule {
   static constexpr int SRC = 3;
   static constexpr int ACC = 6;
   static constexpr bool MASKED_TAIL = false;
+  static constexpr bool ROTATE_LDS = true; // lds_rol1 (see wave_rol1)
 
   static __device__ __forceinline__ void load(const SymArgs &a, int j,
                                               float (&v)[SRC]) {
@@ -283,6 +309,7 @@ struct AdjointRule : PlainPartials<FUSED ? 6 : 3> {
   static constexpr int SRC = 6; // x, y, z, g_x, g_y, g_z
   static constexpr int ACC = FUSED ? 6 : 3; // sum M (g_j - g_k) [, F]
   static constexpr bool MASKED_TAIL = true;
+  static constexpr bool ROTATE_LDS = false; // DPP (see wave_rol1)
 
   // g = lv' + dt lx' (the cotangent the pair sum is taken of)
   static __device__ __forceinline__ void load(const SymArgs &a, int j,
@@ -403,7 +430,15 @@ struct Lane {
 // target validity; sources are valid below index n).  With SELF (a wave's
 // pass over its own tile, with MASK) the self pairs contribute nothing
 // either: needed when eps^3 is not a normal float.
-template <class R, int U, bool SYM, bool MASK, bool SELF = false>
+// PRIO: the wave lowers its issue priority at each quarter of the pass
+// (s_setprio 3, 2, 1, 0), so that the SIMD's arbiter, which otherwise
+// favours the oldest wave, lets the waves that lag behind catch up.  Used by
+// the one-pass tile-pair form at small n, where every wave of the launch
+// starts at once and, without it, finishes between 10 and 37 us into a
+// 40 us kernel (n = 12800, per-wave clock stamps): the last quarter of the
+// launch ran on one or two waves per SIMD.
+template <class R, int U, bool SYM, bool MASK, bool SELF = false,
+          bool PRIO = false>
 __device__ __forceinline__ void tile_pass(Lane<R, U> &L, const float *src,
                                           int j, int lane, float eps,
                                           const bool (&vt)[U], int src0,
@@ -414,6 +449,15 @@ __device__ __forceinline__ void tile_pass(Lane<R, U> &L, const float *src,
   constexpr int Q = U / 4;
   static_assert(U == 2 || U % 4 == 0, "particles per lane: 2 or a multiple of 4");
   for (int s = 0; s < 64; ++s) {
+    if constexpr (PRIO) {
+      if ((s & 15) == 0) {
+        const int quarter = s >> 4;
+        if (quarter == 0) __builtin_amdgcn_s_setprio(3);
+        else if (quarter == 1) __builtin_amdgcn_s_setprio(2);
+        else if (quarter == 2) __builtin_amdgcn_s_setprio(1);
+        else __builtin_amdgcn_s_setprio(0);
+      }
+    }
     const int m = (lane + s) & 63;
     float sv[U][R::SRC];
 #pragma unroll
@@ -451,7 +495,9 @@ __device__ __forceinline__ void tile_pass(Lane<R, U> &L, const float *src,
 #pragma unroll
       for (int q = 0; q < U; ++q)
 #pragma unroll
-        for (int c = 0; c < R::ACC; ++c) L.rs[q][c] = wave_rol1(L.rs[q][c]);
+        for (int c = 0; c < R::ACC; ++c)
+          L.rs[q][c] =
+              R::ROTATE_LDS ? lds_rol1(L.rs[q][c]) : wave_rol1(L.rs[q][c]);
     }
   }
 }
@@ -897,7 +943,7 @@ __global__ __launch_bounds__(256) void sym_band_finish(SymArgs a) {
 // T (T + 1) / 2 wave items fill the chip at sizes where the super-tile items
 // (T / 4)^2 / 2 workgroups of four waves) do not.
 
-template <class R, int U>
+template <class R, int U, bool PRIO = false>
 __global__ __launch_bounds__(256) void sym_tile_kernel(SymArgs a) {
   constexpr int P = 64 * U;
   constexpr int ST = kWaves * P;
@@ -946,23 +992,27 @@ __global__ __launch_bounds__(256) void sym_tile_kernel(SymArgs a) {
   if constexpr (R::MASKED_TAIL) masked = (n % P) != 0 && B == T - 1;
   if (A == B) {
     if (!self_term_vanishes(a.c.eps)) // self pairs masked, as the tail
-      tile_pass<R, U, false, true, true>(L, src, w, lane, a.c.eps, vt, B * P,
+      tile_pass<R, U, false, true, true, PRIO>(L, src, w, lane, a.c.eps, vt, B * P,
                                          n);
     else if (masked)
-      tile_pass<R, U, false, R::MASKED_TAIL>(L, src, w, lane, a.c.eps, vt,
+      tile_pass<R, U, false, R::MASKED_TAIL, false, PRIO>(L, src, w, lane,
+                                                          a.c.eps, vt,
                                              B * P, n);
     else
-      tile_pass<R, U, false, false>(L, src, w, lane, a.c.eps, vt, B * P, n);
+      tile_pass<R, U, false, false, false, PRIO>(L, src, w, lane, a.c.eps,
+                                                 vt, B * P, n);
   } else {
 #pragma unroll
     for (int q = 0; q < U; ++q)
 #pragma unroll
       for (int e = 0; e < R::ACC; ++e) L.rs[q][e] = 0.0f;
     if (masked)
-      tile_pass<R, U, true, R::MASKED_TAIL>(L, src, w, lane, a.c.eps, vt,
+      tile_pass<R, U, true, R::MASKED_TAIL, false, PRIO>(L, src, w, lane,
+                                                         a.c.eps, vt,
                                             B * P, n);
     else
-      tile_pass<R, U, true, false>(L, src, w, lane, a.c.eps, vt, B * P, n);
+      tile_pass<R, U, true, false, false, PRIO>(L, src, w, lane, a.c.eps, vt,
+                                                B * P, n);
     // after 64 rotations lane l holds the sums of source set l again
     float *sp = slot + R::ACC * P + lane * U;
 #pragma unroll
@@ -1463,6 +1513,17 @@ int launch_forward_sym(const StepConsts &c, int unroll, int chunk, bool tiles,
         find_kahan_shape(unroll, tiles), a, unroll, chunk, n, scratch,
         KahanForceRule::ACC, tiles, stream,
         tiles ? nullptr : find_band_shape<KahanForceRule>(unroll));
+  // Tile pairs of 2 per lane up to n = 12800 with the quarter-pass priority
+  // steps (PRIO in tile_pass; same results bit for bit): n = 4096 20.6 vs
+  // 23.8 us, 8192 30.6 vs 31.4 us, 12800 47.4 vs 49.2 us; from 16384 on
+  // (more than five waves per SIMD over the launch) they measured 1-3 %
+  // slower.
+  if (tiles && unroll == 2 && (size_t)n <= kTilePrioMaxParticles) {
+    static const SymShape prio = {2, sym_tile_kernel<ForceRule, 2, true>,
+                                  sym_tile_finalize<ForceRule, 2>};
+    return launch_sym(&prio, a, unroll, chunk, n, scratch, ForceRule::ACC,
+                      tiles, stream, nullptr);
+  }
   return launch_sym(find_shape<ForceRule>(unroll, tiles), a, unroll, chunk, n,
                     scratch, ForceRule::ACC, tiles, stream,
                     tiles ? nullptr : find_band_shape<ForceRule>(unroll));
