@@ -89,10 +89,11 @@ constexpr int kDppWaveRol1 = 0x134;
 //              it is complete and read again only in the next step).  The
 //              forward rules rotate with lds_rol1 (N = 262144: 11.3 vs 11.9
 //              ms with plain sums, 22.5 vs 23.8 ms with Kahan sums; the
-//              64-system batch 4.43 vs 4.52 ms); the adjoint rules keep DPP,
-//              whose loops read twice the LDS per step already (lds_rol1
-//              measured 1.5 % slower for the adjoint, 4 % for the fused
-//              step).
+//              64-system batch 4.43 vs 4.52 ms); the adjoint rules, whose
+//              loops read twice the LDS per step already, keep DPP for the
+//              adjoint sums (lds_rol1 for all of them measured 1.5 % slower
+//              for the adjoint, 4 % for the fused step) and the fused step
+//              rotates its force sums with lds_rol1.
 __device__ __forceinline__ float wave_rol1(float v) {
   const int b = __builtin_bit_cast(int, v);
   return __builtin_bit_cast(
@@ -180,7 +181,8 @@ struct ForceRule : PlainPartials<3> {
   static constexpr int SRC = 3; // x, y, z
   static constexpr int ACC = 3; // F
   static constexpr bool MASKED_TAIL = false;
-  static constexpr bool ROTATE_LDS = true; // lds_rol1 (see wave_rol1)
+  // every sum rotates with lds_rol1 (see wave_rol1)
+  static constexpr bool rotate_lds(int) { return true; }
 
   static __device__ __forceinline__ void load(const SymArgs &a, int j,
                                               float (&v)[SRC]) {
@@ -235,7 +237,8 @@ struct KahanForceRule {
   static constexpr int SRC = 3;
   static constexpr int ACC = 6;
   static constexpr bool MASKED_TAIL = false;
-  static constexpr bool ROTATE_LDS = true; // lds_rol1 (see wave_rol1)
+  // every sum rotates with lds_rol1 (see wave_rol1)
+  static constexpr bool rotate_lds(int) { return true; }
 
   static __device__ __forceinline__ void load(const SymArgs &a, int j,
                                               float (&v)[SRC]) {
@@ -309,7 +312,10 @@ struct AdjointRule : PlainPartials<FUSED ? 6 : 3> {
   static constexpr int SRC = 6; // x, y, z, g_x, g_y, g_z
   static constexpr int ACC = FUSED ? 6 : 3; // sum M (g_j - g_k) [, F]
   static constexpr bool MASKED_TAIL = true;
-  static constexpr bool ROTATE_LDS = false; // DPP (see wave_rol1)
+  // the adjoint sums rotate by DPP; the fused step's force sums (3..5) with
+  // lds_rol1 (N = 262144: 24.4 vs 25.1 ms; all six through the crossbar: 26.1
+  // ms; any of the plain adjoint's through it: no gain)
+  static constexpr bool rotate_lds(int c) { return FUSED && c >= 3; }
 
   // g = lv' + dt lx' (the cotangent the pair sum is taken of)
   static __device__ __forceinline__ void load(const SymArgs &a, int j,
@@ -497,7 +503,7 @@ __device__ __forceinline__ void tile_pass(Lane<R, U> &L, const float *src,
 #pragma unroll
         for (int c = 0; c < R::ACC; ++c)
           L.rs[q][c] =
-              R::ROTATE_LDS ? lds_rol1(L.rs[q][c]) : wave_rol1(L.rs[q][c]);
+              R::rotate_lds(c) ? lds_rol1(L.rs[q][c]) : wave_rol1(L.rs[q][c]);
     }
   }
 }
