@@ -238,6 +238,29 @@ def test_forward_configs_are_described(nb):
     assert lib.nb_forward_config_scheme(n) == -1 and lib.nb_forward_config_scheme(-1) == -1
 
 
+def test_default_shapes_are_reported(nb):
+    """The shapes cfg = -1 picks (host logic, no GPU): symmetric 8 per lane at
+    N = 262144, tile pairs of 2 per lane at the reference's default 12800,
+    target-sharded below 4096, for PREDICATED, and for a NULL params."""
+    info = lambda c: nb.forward_config_info(c)
+    big = info(nb.forward_default_config(262144))
+    assert (big["scheme"], big["unroll"], big["split"]) == ("symmetric", 8, 1)
+    small = info(nb.forward_default_config(12800))
+    assert (small["scheme"], small["unroll"]) == ("symmetric_tiles", 2)
+    assert info(nb.forward_default_config(1000))["scheme"] == "targets"
+    pred = nb.default_params(num_particles=262144, calc_method=nb.NB_CALC_PREDICATED)
+    assert info(nb.forward_default_config(262144, pred))["scheme"] == "targets"
+    kahan = info(nb.forward_default_config(262144, acc=nb.NB_ACC_KAHAN))
+    assert (kahan["scheme"], kahan["unroll"]) == ("symmetric", 2)
+    adj = nb.adjoint_config_info(nb.adjoint_default_config(262144))
+    assert (adj["scheme"], adj["unroll"]) == ("symmetric", 2)
+    lib = nb.load()
+    assert lib.nb_forward_default_config(None, 100, 0) == -1
+    assert lib.nb_adjoint_default_config(None, 100) == -1
+    with pytest.raises(nb.NbError):
+        nb.forward_default_config(100, acc=7)
+
+
 def test_adjoint_configs_are_described(nb):
     n = nb.adjoint_config_count()
     schemes = [nb.adjoint_config_info(c)["scheme"] for c in range(n)]
