@@ -125,11 +125,16 @@ extern "C" int nb_scratch_release(void) {
   int dev = 0;
   int st = hip_check(hipGetDevice(&dev), "hipGetDevice");
   if (st) return st;
-  if ((st = hip_check(hipDeviceSynchronize(), "device synchronize"))) return st;
+  // Every launch that uses a buffer is enqueued under its entry's lock: with
+  // all of the device's entry locks held no new one can be, and the device
+  // synchronisation then waits for those already enqueued.
   std::lock_guard<std::mutex> lock(g_scratch_mu);
+  std::vector<std::unique_lock<std::mutex>> held;
+  for (auto &e : g_scratch)
+    if (e->device == dev) held.emplace_back(e->mu);
+  if ((st = hip_check(hipDeviceSynchronize(), "device synchronize"))) return st;
   for (auto &e : g_scratch) {
     if (e->device != dev) continue;
-    std::lock_guard<std::mutex> entry_lock(e->mu);
     if (e->ptr) (void)hipFree(e->ptr);
     e->ptr = nullptr;
     e->bytes = 0;
