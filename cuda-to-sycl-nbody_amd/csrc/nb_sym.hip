@@ -436,6 +436,8 @@ struct Lane {
 // target validity; sources are valid below index n).  With SELF (a wave's
 // pass over its own tile, with MASK) the self pairs contribute nothing
 // either: needed when eps^3 is not a normal float.
+// [s0, s1): the steps of the pass this call runs (a half pass leaves lane l
+// with the reaction sums of set (l + s1) mod 64).
 // PRIO: the wave lowers its issue priority at each quarter of the pass
 // (s_setprio 3, 2, 1, 0), so that the SIMD's arbiter, which otherwise
 // favours the oldest wave, lets the waves that lag behind catch up.  Used by
@@ -448,16 +450,17 @@ template <class R, int U, bool SYM, bool MASK, bool SELF = false,
 __device__ __forceinline__ void tile_pass(Lane<R, U> &L, const float *src,
                                           int j, int lane, float eps,
                                           const bool (&vt)[U], int src0,
-                                          int n) {
+                                          int n, int s0 = 0, int s1 = 64) {
   static_assert(!SELF || (MASK && !SYM), "self pairs: masked own-tile pass");
   constexpr int P = 64 * U;
   constexpr int ST = kWaves * P;
   constexpr int Q = U / 4;
   static_assert(U == 2 || U % 4 == 0, "particles per lane: 2 or a multiple of 4");
-  for (int s = 0; s < 64; ++s) {
+  for (int s = s0; s < s1; ++s) {
     if constexpr (PRIO) {
-      if ((s & 15) == 0) {
-        const int quarter = s >> 4;
+      const int qlen = (s1 - s0) >> 2; // 16 or 8 steps
+      if (((s - s0) & (qlen - 1)) == 0) {
+        const int quarter = (s - s0) / qlen;
         if (quarter == 0) __builtin_amdgcn_s_setprio(3);
         else if (quarter == 1) __builtin_amdgcn_s_setprio(2);
         else if (quarter == 2) __builtin_amdgcn_s_setprio(1);
@@ -949,27 +952,35 @@ __global__ __launch_bounds__(256) void sym_band_finish(SymArgs a) {
 // T (T + 1) / 2 wave items fill the chip at sizes where the super-tile items
 // (T / 4)^2 / 2 workgroups of four waves) do not.
 
-template <class R, int U, bool PRIO = false>
+// HALVES = 2 splits each tile pair into two work items of 32 steps (the
+// first and second half of the pass; item = 2 x pair + half), each with its
+// own row and reaction partial: twice the items, so that at small n the
+// launch holds more items than resident waves and the dispatcher keeps every
+// SIMD fed until a short tail, for twice the partials in the finalize.
+template <class R, int U, bool PRIO = false, int HALVES = 1>
 __global__ __launch_bounds__(256) void sym_tile_kernel(SymArgs a) {
+  static_assert(HALVES == 1 || HALVES == 2, "one or two items per tile pair");
   constexpr int P = 64 * U;
   constexpr int ST = kWaves * P;
   __shared__ float4 src4[R::SRC][ST / 4]; // wave w's source tile is tile w
   float *src = reinterpret_cast<float *>(src4);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = a.n, T = a.ts;
-  const long long items = (long long)T * (T + 1) / 2;
+  const long long items = (long long)HALVES * T * (T + 1) / 2;
   const long long item = (long long)blockIdx.x * kWaves + w;
   const bool live = item < items;
+  const long long pair = item / HALVES;
+  const int s0 = (int)(item % HALVES) * (64 / HALVES), s1 = s0 + 64 / HALVES;
   int A = 0, B = 0;
-  if (live) { // row A: the last row whose first slot is <= item
+  if (live) { // row A: the last row whose first slot is <= pair
     int lo = 0, hi = T - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (tri_slot(mid, mid, T) <= item) lo = mid;
+      if (tri_slot(mid, mid, T) <= pair) lo = mid;
       else hi = mid - 1;
     }
     A = lo;
-    B = A + (int)(item - tri_slot(A, A, T));
+    B = A + (int)(pair - tri_slot(A, A, T));
   }
   // targets first: their loads are in flight while the source tile is staged
   Lane<R, U> L;
@@ -999,14 +1010,14 @@ __global__ __launch_bounds__(256) void sym_tile_kernel(SymArgs a) {
   if (A == B) {
     if (!self_term_vanishes(a.c.eps)) // self pairs masked, as the tail
       tile_pass<R, U, false, true, true, PRIO>(L, src, w, lane, a.c.eps, vt, B * P,
-                                         n);
+                                         n, s0, s1);
     else if (masked)
       tile_pass<R, U, false, R::MASKED_TAIL, false, PRIO>(L, src, w, lane,
                                                           a.c.eps, vt,
-                                             B * P, n);
+                                             B * P, n, s0, s1);
     else
       tile_pass<R, U, false, false, false, PRIO>(L, src, w, lane, a.c.eps,
-                                                 vt, B * P, n);
+                                                 vt, B * P, n, s0, s1);
   } else {
 #pragma unroll
     for (int q = 0; q < U; ++q)
@@ -1015,12 +1026,13 @@ __global__ __launch_bounds__(256) void sym_tile_kernel(SymArgs a) {
     if (masked)
       tile_pass<R, U, true, R::MASKED_TAIL, false, PRIO>(L, src, w, lane,
                                                          a.c.eps, vt,
-                                            B * P, n);
+                                            B * P, n, s0, s1);
     else
       tile_pass<R, U, true, false, false, PRIO>(L, src, w, lane, a.c.eps, vt,
-                                                B * P, n);
-    // after 64 rotations lane l holds the sums of source set l again
-    float *sp = slot + R::ACC * P + lane * U;
+                                                B * P, n, s0, s1);
+    // after the rotations of steps [s0, s1) lane l holds the sums of source
+    // set (l + s1) mod 64 (set l again after a whole pass)
+    float *sp = slot + R::ACC * P + ((lane + s1) & 63) * U;
 #pragma unroll
     for (int q = 0; q < U; ++q)
 #pragma unroll
@@ -1036,7 +1048,7 @@ __global__ __launch_bounds__(256) void sym_tile_kernel(SymArgs a) {
 // Per particle: row partials of its tile (B order) minus the negated
 // reaction partials of its column (A order), in kTileRuns runs; then the
 // rule's epilogue.
-template <class R, int U>
+template <class R, int U, int HALVES = 1>
 __global__ __launch_bounds__(256) void sym_tile_finalize(SymArgs a) {
   constexpr int P = 64 * U;
   constexpr int PW = 256 / kTileRuns; // particles per workgroup
@@ -1047,14 +1059,17 @@ __global__ __launch_bounds__(256) void sym_tile_finalize(SymArgs a) {
   float f[R::ACC];
   if (live) {
     const int g = i / P, o = i % P, T = a.ts;
+    // rows of pairs (g, g + k / HALVES), then the reactions of (c / HALVES,
+    // g), each pair's halves in order
     run_sum<R, kTileRuns>(
-        f, T - g, g, run,
+        f, HALVES * (T - g), HALVES * g, run,
         [&](int k) {
-          return a.fpart + (size_t)tri_slot(g, g + k, T) * (2 * R::ACC * P) + o;
+          const long long it = tri_slot(g, g + k / HALVES, T) * HALVES + k % HALVES;
+          return a.fpart + (size_t)it * (2 * R::ACC * P) + o;
         },
         [&](int c) {
-          return a.fpart + (size_t)tri_slot(c, g, T) * (2 * R::ACC * P) +
-                 R::ACC * P + o;
+          const long long it = tri_slot(c / HALVES, g, T) * HALVES + c % HALVES;
+          return a.fpart + (size_t)it * (2 * R::ACC * P) + R::ACC * P + o;
         },
         P);
   } else {
@@ -1355,13 +1370,16 @@ int launch_sym(const SymShape *sh, SymArgs &a, int unroll, int chunk, int n,
     return set_error(NB_ERR_INVALID_ARG, "symmetric launch needs scratch");
   long long blocks;
   if (tiles) {
+    if (chunk > 2)
+      return set_error(NB_ERR_UNSUPPORTED,
+                       "tile-pair shapes take one or two items per tile pair");
     const int P = 64 * unroll;
     const int T = (n + P - 1) / P;
     a.fpart = scratch;
     a.spart = nullptr;
     a.ts = T;
-    a.chunk = 1;
-    blocks = ((long long)T * (T + 1) / 2 + kWaves - 1) / kWaves;
+    a.chunk = chunk; // items per tile pair
+    blocks = ((long long)chunk * T * (T + 1) / 2 + kWaves - 1) / kWaves;
   } else {
     const int st = kWaves * 64 * unroll;
     const int ts = (n + st - 1) / st;
@@ -1394,7 +1412,8 @@ size_t sym_scratch_bytes(int unroll, int chunk, size_t n, int acc,
     if (n == 0 || n > sym_max_particles(unroll, tiles, chunk, acc)) return 0;
     const long long P = 64LL * unroll;
     const long long T = ((long long)n + P - 1) / P;
-    return sizeof(float) * (size_t)(T * (T + 1) / 2) * 2 * acc * (size_t)P;
+    return sizeof(float) * (size_t)(T * (T + 1) / 2) * 2 * acc * (size_t)P *
+           (size_t)std::max(chunk, 1);
   }
   if (bandable && sym_banded(unroll, tiles, chunk, acc, n)) {
     const BandPlan bp = band_plan(n, unroll, chunk);
@@ -1514,6 +1533,9 @@ int launch_forward_sym(const StepConsts &c, int unroll, int chunk, bool tiles,
   a.ny = ny;
   a.nz = nz;
   a.c = c;
+  if (tiles && chunk != 1 && (kahan || unroll != 2))
+    return set_error(NB_ERR_UNSUPPORTED,
+                     "two items per tile pair: plain sums, 2 per lane only");
   if (kahan)
     return launch_sym(
         find_kahan_shape(unroll, tiles), a, unroll, chunk, n, scratch,
@@ -1524,6 +1546,16 @@ int launch_forward_sym(const StepConsts &c, int unroll, int chunk, bool tiles,
   // 23.8 us, 8192 30.6 vs 31.4 us, 12800 47.4 vs 49.2 us; from 16384 on
   // (more than five waves per SIMD over the launch) they measured 1-3 %
   // slower.
+  // Two items per tile pair (forward config 37, the default at 4096 <= n <=
+  // 8192): n = 4096 15.6 vs 20.4 us, 8192 28.8-29.3 vs 29.8-30.2 us; from
+  // 12800 on the doubled partials cost more than the balance gains (12800
+  // 47.2 vs 42.5 us).  Priority steps on the half items measured slower.
+  if (tiles && unroll == 2 && chunk == 2) {
+    static const SymShape halves = {2, sym_tile_kernel<ForceRule, 2, false, 2>,
+                                    sym_tile_finalize<ForceRule, 2, 2>};
+    return launch_sym(&halves, a, unroll, chunk, n, scratch, ForceRule::ACC,
+                      tiles, stream, nullptr);
+  }
   if (tiles && unroll == 2 && (size_t)n <= kTilePrioMaxParticles) {
     static const SymShape prio = {2, sym_tile_kernel<ForceRule, 2, true>,
                                   sym_tile_finalize<ForceRule, 2>};

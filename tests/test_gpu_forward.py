@@ -242,6 +242,16 @@ def test_frame_timing(nb, gpu):
     assert "gfx950" in nb.device_name(), nb.device_name()
 
 
+def has_kahan(info):
+    """The shape has a Kahan variant: every target-sharded shape, the
+    symmetric ones with 2 or 4 particles per lane and one item per tile pair
+    (the two-item tile-pair shape is plain sums only)."""
+    if info["scheme"] == "targets":
+        return True
+    return info["unroll"] <= 4 and not (info["scheme"] == "symmetric_tiles"
+                                        and info["split"] > 1)
+
+
 def sym_configs(nb):
     """Symmetric shapes: super-tile items first, then the tile-pair form."""
     info = [nb.forward_config_info(c) for c in range(nb.forward_config_count())]
@@ -541,7 +551,7 @@ def test_symmetric_scheme_vanishing_eps(nb, orc, gpu, eps, n):
     rp, _ = orc.step_f32(p, pos, vel)
     for cfg in sym_configs(nb):
         accs = [nb.NB_ACC_PLAIN]
-        if nb.forward_config_info(cfg)["unroll"] <= 4:
+        if has_kahan(nb.forward_config_info(cfg)):
             accs.append(nb.NB_ACC_KAHAN)
         for acc in accs:
             dp, dv = dev3(torch, pos, gpu), dev3(torch, vel, gpu)
@@ -588,7 +598,7 @@ def test_coincident_particles(nb, orc, gpu, eps):
     for cfg in range(nb.forward_config_count()):
         info = nb.forward_config_info(cfg)
         accs = [nb.NB_ACC_PLAIN]
-        if info["scheme"] == "targets" or info["unroll"] <= 4:
+        if has_kahan(info):
             accs.append(nb.NB_ACC_KAHAN)
         for acc in accs:
             dp, dv = dev3(torch, pos, gpu), dev3(torch, vel, gpu)
@@ -660,6 +670,15 @@ def test_symmetric_scheme_rejects_other_modes(nb, gpu):
         dn = [torch.empty_like(t) for t in dp]
         with pytest.raises(nb.NbError, match="symmetric"):
             nb.forward_step(params, dp, dv, dn, cfg=cfg, acc=acc)
+    # the two-item tile-pair shape takes plain sums only
+    two = [c for c in sym_configs(nb) if not has_kahan(nb.forward_config_info(c))
+           and nb.forward_config_info(c)["scheme"] == "symmetric_tiles"]
+    assert two
+    params = nb.default_params(num_particles=n)
+    dp, dv = dev3(torch, pos, gpu), dev3(torch, vel, gpu)
+    dn = [torch.empty_like(t) for t in dp]
+    with pytest.raises(nb.NbError, match="two items per tile pair"):
+        nb.forward_step(params, dp, dv, dn, cfg=two[0], acc=nb.NB_ACC_KAHAN)
     # the simulator's default shape follows the accumulation mode
     sim = nb.Simulator(nb.default_params(num_particles=70000), *nb.disk_ic(70000))
     assert nb.forward_config_info(sim.config)["scheme"] != "targets"
@@ -773,7 +792,7 @@ def test_symmetric_kahan(nb, orc, gpu, n):
     idx = np.arange(0, n, 7)
     f64 = np.stack(orc.forces_f64(pos, idx))
     unit = nb.default_params(num_particles=n, G=1.0, dt=1.0, damping=0.0)
-    kahan_cfgs = [c for c in sym_configs(nb) if nb.forward_config_info(c)["unroll"] <= 4]
+    kahan_cfgs = [c for c in sym_configs(nb) if has_kahan(nb.forward_config_info(c))]
     assert kahan_cfgs
     for cfg in kahan_cfgs:
         errs = {}
