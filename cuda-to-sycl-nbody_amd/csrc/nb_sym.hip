@@ -456,17 +456,7 @@ __device__ __forceinline__ void tile_pass(Lane<R, U> &L, const float *src,
   constexpr int ST = kWaves * P;
   constexpr int Q = U / 4;
   static_assert(U == 2 || U % 4 == 0, "particles per lane: 2 or a multiple of 4");
-  for (int s = s0; s < s1; ++s) {
-    if constexpr (PRIO) {
-      const int qlen = (s1 - s0) >> 2; // 16 or 8 steps
-      if (((s - s0) & (qlen - 1)) == 0) {
-        const int quarter = (s - s0) / qlen;
-        if (quarter == 0) __builtin_amdgcn_s_setprio(3);
-        else if (quarter == 1) __builtin_amdgcn_s_setprio(2);
-        else if (quarter == 2) __builtin_amdgcn_s_setprio(1);
-        else __builtin_amdgcn_s_setprio(0);
-      }
-    }
+  auto step = [&](int s) {
     const int m = (lane + s) & 63;
     float sv[U][R::SRC];
 #pragma unroll
@@ -508,6 +498,20 @@ __device__ __forceinline__ void tile_pass(Lane<R, U> &L, const float *src,
           L.rs[q][c] =
               R::rotate_lds(c) ? lds_rol1(L.rs[q][c]) : wave_rol1(L.rs[q][c]);
     }
+  };
+  if constexpr (PRIO) {
+    // the priority is set once per quarter, outside the step loop
+    const int qlen = (s1 - s0) >> 2; // 16 or 8 steps
+    for (int quarter = 0; quarter < 4; ++quarter) {
+      if (quarter == 0) __builtin_amdgcn_s_setprio(3);
+      else if (quarter == 1) __builtin_amdgcn_s_setprio(2);
+      else if (quarter == 2) __builtin_amdgcn_s_setprio(1);
+      else __builtin_amdgcn_s_setprio(0);
+      const int e = s0 + (quarter + 1) * qlen;
+      for (int s = s0 + quarter * qlen; s < e; ++s) step(s);
+    }
+  } else {
+    for (int s = s0; s < s1; ++s) step(s);
   }
 }
 
