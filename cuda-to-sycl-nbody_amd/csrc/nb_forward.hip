@@ -175,7 +175,8 @@ int forward_default_config(size_t n, int mode) {
   if (mode == MODE_PLAIN_SKIP) mode = MODE_PLAIN;
   if (mode == MODE_KAHAN_SKIP) mode = MODE_KAHAN;
   // Measured on MI355X (disk ICs), us per step.  Tile pairs, 2 per lane
-  // (two items per tile pair up to 8192: n = 4096 15.6, 8192 28.8; see
+  // (two items per tile pair below 6656: n = 4096 15.5, 6144 20.9; one item
+  // with priority steps from there: 7168 23.1, 8192 27.2; see
   // launch_forward_sym): n = 4096 22.6 (target-sharded 128 x 1 split 8: 29), 8192 38.1 (38.7),
   // 12800 57.5 (82.5), 32768 238.4 (323.6; 4 per lane 266.3).  Super-tile
   // pairs, 2 per lane, 1 / 2 per item: 49152 481.8 / 532.7 (tile pairs
@@ -193,7 +194,7 @@ int forward_default_config(size_t n, int mode) {
     if (n >= ((size_t)1 << 18)) return 22; // symmetric, 8 per lane, 1 per item
     if (n >= 98304) return 35;             // symmetric, 2 per lane, 2 per item
     if (n >= 49152) return 34;             // symmetric, 2 per lane, 1 per item
-    if (n > 8192) return 33;               // symmetric tile pairs, 2 per lane
+    if (n >= 6656) return 33;              // symmetric tile pairs, 2 per lane
     if (n >= 4096) return 37;              // the same, two items per tile pair
   }
   // Kahan, Gint/s: tile pairs, 2 per lane: n = 16384 1961 (best

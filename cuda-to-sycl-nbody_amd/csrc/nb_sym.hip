@@ -1550,9 +1550,10 @@ int launch_forward_sym(const StepConsts &c, int unroll, int chunk, bool tiles,
   // 23.8 us, 8192 30.6 vs 31.4 us, 12800 47.4 vs 49.2 us; from 16384 on
   // (more than five waves per SIMD over the launch) they measured 1-3 %
   // slower.
-  // Two items per tile pair (forward config 37, the default at 4096 <= n <=
-  // 8192): n = 4096 15.6 vs 20.4 us, 8192 28.8-29.3 vs 29.8-30.2 us; from
-  // 12800 on the doubled partials cost more than the balance gains (12800
+  // Two items per tile pair (forward config 37, the default at 4096 <= n <
+  // 6656) against one item with priority steps: n = 4096 15.5 vs 20.0 us,
+  // 6144 20.9 vs 22.2 us; from 7168 on the doubled partials cost more than
+  // the balance gains (7168 24.6 vs 23.1 us, 8192 28.6 vs 27.2 us, 12800
   // 47.2 vs 42.5 us).  Priority steps on the half items measured slower.
   if (tiles && unroll == 2 && chunk == 2) {
     static const SymShape halves = {2, sym_tile_kernel<ForceRule, 2, false, 2>,

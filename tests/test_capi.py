@@ -247,6 +247,10 @@ def test_default_shapes_are_reported(nb):
     assert (big["scheme"], big["unroll"], big["split"]) == ("symmetric", 8, 1)
     small = info(nb.forward_default_config(12800))
     assert (small["scheme"], small["unroll"]) == ("symmetric_tiles", 2)
+    # two items per tile pair below 6656 particles, one from there
+    for n, split in ((4096, 2), (6655, 2), (6656, 1), (8192, 1)):
+        tp = info(nb.forward_default_config(n))
+        assert (tp["scheme"], tp["unroll"], tp["split"]) == ("symmetric_tiles", 2, split), n
     assert info(nb.forward_default_config(1000))["scheme"] == "targets"
     pred = nb.default_params(num_particles=262144, calc_method=nb.NB_CALC_PREDICATED)
     assert info(nb.forward_default_config(262144, pred))["scheme"] == "targets"
