@@ -72,7 +72,7 @@ namespace nb {
 namespace {
 
 constexpr int kWaves = 4;        // waves per workgroup = tiles per super-tile
-constexpr size_t kTilePrioMaxParticles = 12800; // see launch_forward_sym
+constexpr size_t kTilePrioMaxParticles = 32768; // see launch_forward_sym
 constexpr float kFar = 1.0e30f;  // coordinate of ForceRule padding particles
 constexpr int kDppWaveRol1 = 0x134;
 
@@ -1545,11 +1545,12 @@ int launch_forward_sym(const StepConsts &c, int unroll, int chunk, bool tiles,
         find_kahan_shape(unroll, tiles), a, unroll, chunk, n, scratch,
         KahanForceRule::ACC, tiles, stream,
         tiles ? nullptr : find_band_shape<KahanForceRule>(unroll));
-  // Tile pairs of 2 per lane up to n = 12800 with the quarter-pass priority
+  // Tile pairs of 2 per lane up to n = 32768 with the quarter-pass priority
   // steps (PRIO in tile_pass; same results bit for bit): n = 4096 20.6 vs
-  // 23.8 us, 8192 30.6 vs 31.4 us, 12800 47.4 vs 49.2 us; from 16384 on
-  // (more than five waves per SIMD over the launch) they measured 1-3 %
-  // slower.
+  // 23.8 us, 8192 30.6 vs 31.4 us, 12800 47.4 vs 49.2 us; with the priority
+  // set outside the step loop also 14336 56.5-57.5 vs 59.6-60.3 us, 20480
+  // 94.5-96.8 vs 97.2-98.5 us, 25600 and 32768 about 1 % faster, 16384 the
+  // same, 40960 0.3 % slower.
   // Two items per tile pair (forward config 37, the default at 4096 <= n <
   // 6656) against one item with priority steps: n = 4096 15.5 vs 20.0 us,
   // 6144 20.9 vs 22.2 us; from 7168 on the doubled partials cost more than
