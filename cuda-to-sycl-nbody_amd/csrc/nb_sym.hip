@@ -1555,9 +1555,11 @@ int launch_forward_sym(const StepConsts &c, int unroll, int chunk, bool tiles,
   // 6656) against one item with priority steps: n = 4096 15.5 vs 20.0 us,
   // 6144 20.9 vs 22.2 us; from 7168 on the doubled partials cost more than
   // the balance gains (7168 24.6 vs 23.1 us, 8192 28.6 vs 27.2 us, 12800
-  // 47.2 vs 42.5 us).  Priority steps on the half items measured slower.
+  // 47.2 vs 42.5 us).  The half items take the priority steps as well (per
+  // quarter of their 32 steps): 4096 15.1-15.4 vs 15.4-15.5 us, 5120
+  // 16.5-16.8 vs 17.0-17.1 us, 6144 19.8-19.9 vs 20.9-21.0 us.
   if (tiles && unroll == 2 && chunk == 2) {
-    static const SymShape halves = {2, sym_tile_kernel<ForceRule, 2, false, 2>,
+    static const SymShape halves = {2, sym_tile_kernel<ForceRule, 2, true, 2>,
                                     sym_tile_finalize<ForceRule, 2, 2>};
     return launch_sym(&halves, a, unroll, chunk, n, scratch, ForceRule::ACC,
                       tiles, stream, nullptr);
