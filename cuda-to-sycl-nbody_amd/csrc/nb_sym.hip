@@ -441,10 +441,11 @@ struct Lane {
 // PRIO: the wave lowers its issue priority at each quarter of the pass
 // (s_setprio 3, 2, 1, 0), so that the SIMD's arbiter, which otherwise
 // favours the oldest wave, lets the waves that lag behind catch up.  Used by
-// the one-pass tile-pair form at small n, where every wave of the launch
-// starts at once and, without it, finishes between 10 and 37 us into a
-// 40 us kernel (n = 12800, per-wave clock stamps): the last quarter of the
-// launch ran on one or two waves per SIMD.
+// the tile-pair forms (forward up to n = 32768, adjoint and two-item forms
+// at every size they run), where most waves of the launch start at once
+// and, without it, finish between 10 and 37 us into a 40 us kernel (n =
+// 12800, per-wave clock stamps): the last quarter of the launch ran on one
+// or two waves per SIMD.
 template <class R, int U, bool SYM, bool MASK, bool SELF = false,
           bool PRIO = false>
 __device__ __forceinline__ void tile_pass(Lane<R, U> &L, const float *src,
