@@ -73,6 +73,7 @@ namespace {
 
 constexpr int kWaves = 4;        // waves per workgroup = tiles per super-tile
 constexpr size_t kTilePrioMaxParticles = 32768; // see launch_forward_sym
+constexpr size_t kKahanTilePrioMaxParticles = 16384; // the same, Kahan sums
 constexpr float kFar = 1.0e30f;  // coordinate of ForceRule padding particles
 constexpr int kDppWaveRol1 = 0x134;
 
@@ -441,8 +442,8 @@ struct Lane {
 // PRIO: the wave lowers its issue priority at each quarter of the pass
 // (s_setprio 3, 2, 1, 0), so that the SIMD's arbiter, which otherwise
 // favours the oldest wave, lets the waves that lag behind catch up.  Used by
-// the tile-pair forms (forward up to n = 32768, adjoint and two-item forms
-// at every size they run), where most waves of the launch start at once
+// the tile-pair forms (forward up to n = 32768, Kahan sums up to 16384,
+// adjoint and two-item forms at every size they run), where most waves of the launch start at once
 // and, without it, finish between 10 and 37 us into a 40 us kernel (n =
 // 12800, per-wave clock stamps): the last quarter of the launch ran on one
 // or two waves per SIMD.
@@ -1541,6 +1542,15 @@ int launch_forward_sym(const StepConsts &c, int unroll, int chunk, bool tiles,
   if (tiles && chunk != 1 && (kahan || unroll != 2))
     return set_error(NB_ERR_UNSUPPORTED,
                      "two items per tile pair: plain sums, 2 per lane only");
+  // Kahan tile pairs with the priority steps up to n = 16384 (same results
+  // bit for bit): 12288 76.2-76.5 vs 79.8-80.0 us, 16384 118.8-118.9 vs
+  // 119.4-119.5 us; from 20480 on 2-5 % slower (more work per wave).
+  if (kahan && tiles && unroll == 2 && (size_t)n <= kKahanTilePrioMaxParticles) {
+    static const SymShape kprio = {2, sym_tile_kernel<KahanForceRule, 2, true>,
+                                   sym_tile_finalize<KahanForceRule, 2>};
+    return launch_sym(&kprio, a, unroll, chunk, n, scratch, KahanForceRule::ACC,
+                      tiles, stream, nullptr);
+  }
   if (kahan)
     return launch_sym(
         find_kahan_shape(unroll, tiles), a, unroll, chunk, n, scratch,
